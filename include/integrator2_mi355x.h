@@ -1,0 +1,149 @@
+/*
+ * integrator2_mi355x.h -- C ABI of the MI355X (gfx950) panel-pair integrator.
+ *
+ * Plain pointers and sizes only; host pointers are borrowed for the duration
+ * of a call, device memory is owned by the objects.  Every function returns
+ * I2_OK (0) or a negative status and never exits the process; the message of
+ * the last failure on the calling thread is returned by i2_last_error().
+ *
+ * Each entry point names the reference interface it replaces
+ * (andreyypopov/integrator2, paths relative to the repository root).
+ */
+#ifndef INTEGRATOR2_MI355X_H
+#define INTEGRATOR2_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference calls exit(EXIT_FAILURE) instead,
+ *      src/common/cuda_helper.cuh:41-54) */
+#define I2_OK 0
+#define I2_ERR_INVALID (-1)  /* bad argument */
+#define I2_ERR_HIP (-2)      /* HIP runtime error or no device */
+#define I2_ERR_ALLOC (-3)    /* device or host allocation failed */
+#define I2_ERR_STATE (-4)    /* call out of order (e.g. results before a run) */
+#define I2_ERR_RANGE (-5)    /* size beyond a documented limit */
+#define I2_ERR_IO (-6)       /* file could not be read or written */
+#define I2_ERR_TOPOLOGY (-7) /* mesh topology the integrator cannot handle */
+
+/* ---- neighbour types: the values of neighbour_type_enum (src/Mesh3d.cuh:18-23) */
+#define I2_SIMPLE_NEIGHBORS 0   /* exactly one shared vertex ("vertex-sharing") */
+#define I2_ATTACHED_NEIGHBORS 1 /* a shared edge ("edge-sharing") */
+#define I2_NOT_NEIGHBORS 2      /* no shared vertex ("disjoint") */
+
+/* ---- output formats: output_format_enum (src/evaluators/evaluator3d.cuh:19-22) */
+#define I2_OUTPUT_PLAIN_TEXT 1
+#define I2_OUTPUT_CSV 2
+
+typedef struct i2_mesh i2_mesh;             /* Mesh3D */
+typedef struct i2_integrator i2_integrator; /* NumericalIntegrator3D */
+typedef struct i2_evaluator i2_evaluator;   /* Evaluator3D / EvaluatorJ3DK */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* i2_last_error(void);
+/* Number of visible HIP devices (0 without a GPU). */
+int i2_device_count(int* n);
+
+/* ======================================================================
+ * Mesh3D (src/Mesh3d.cuh:55-126, src/Mesh3d.cu)
+ * ==================================================================== */
+int i2_mesh_create(i2_mesh** out, int device);
+void i2_mesh_destroy(i2_mesh* m);
+/* Mesh3D::loadMeshFromFile (src/Mesh3d.cu:159-217): "nV nC", then
+ * "idx x y z" per vertex, then "idx 203 a b c" triangles (1-based indices);
+ * entities of other types are skipped.  Coordinates are multiplied by scale. */
+int i2_mesh_load_file(i2_mesh* m, const char* path, double scale);
+/* Direct upload of vertices (xyz, nV*3 doubles) and 0-based triangles (nT*3). */
+int i2_mesh_set_data(i2_mesh* m, const double* xyz, int64_t nV, const int32_t* tri, int64_t nT);
+/* Mesh3D::prepareMesh (src/Mesh3d.cu:219-226): normals, areas and the three
+ * neighbour lists (pairs i<j classified by the number of shared vertex indices). */
+int i2_mesh_prepare(i2_mesh* m);
+/* Sizes; pairs[t] = number of unordered (i<j) pairs of neighbour type t. */
+int i2_mesh_sizes(const i2_mesh* m, int64_t* nV, int64_t* nT, int64_t pairs[3]);
+/* Mesh3D getters (src/Mesh3d.cuh:70-126), copied to host buffers. */
+int i2_mesh_copy_vertices(const i2_mesh* m, double* xyz);
+int i2_mesh_copy_cells(const i2_mesh* m, int32_t* tri);
+int i2_mesh_copy_normals(const i2_mesh* m, double* n3);
+int i2_mesh_copy_measures(const i2_mesh* m, double* s);
+/* getSimpleNeighbors / getAttachedNeighbors / getNotNeighbors: (i, j, k)
+ * with i < j and k the position in the list, lexicographic order. */
+int i2_mesh_copy_neighbors(const i2_mesh* m, int type, int32_t* ijk);
+/* Device pointer and length of a mesh array for deviceVector-style getters:
+ * what = 0 vertices (double3), 1 cells (int3), 2 normals (double3),
+ * 3 measures (double), 4 simple, 5 attached neighbours (int3). */
+int i2_mesh_device_array(const i2_mesh* m, int what, const void** dptr, int64_t* n);
+
+/* ======================================================================
+ * NumericalIntegrator3D (src/NumericalIntegrator3d.cuh:89-327)
+ * ==================================================================== */
+/* NumericalIntegrator3D(mesh, qf) (src/NumericalIntegrator3d.cu:197-216):
+ * L12 = nPts (L1, L2) pairs, w = nPts weights, order = rule order (the Runge
+ * factor is 2^order).  Default mode: automatic error control with the
+ * reference's maxLevel 5 and tolerance 1e-5 (src/common/constants.h:18,56). */
+int i2_integrator_create(i2_integrator** out, const i2_mesh* mesh, const double* L12, const double* w, int nPts, int order);
+void i2_integrator_destroy(i2_integrator* it);
+/* setFixedRefinementLevel (src/NumericalIntegrator3d.cu:225-229).  Level L
+ * integrates over all 4^L sub-panels of each control panel. */
+int i2_integrator_set_fixed_refinement_level(i2_integrator* it, int level);
+/* Automatic error control with run-time maxLevel and Runge tolerance (compile-time
+ * MAX_REFINE_LEVEL / EPS_INTEGRATION in the reference).  refParityQuirk != 0
+ * reproduces the reference's result-buffer swap (src/evaluators/evaluatorJ3DK.cu:976-990). */
+int i2_integrator_set_error_control(i2_integrator* it, int maxLevel, double tol, int refParityQuirk);
+
+/* ======================================================================
+ * Evaluator3D / EvaluatorJ3DK (src/evaluators/evaluator3d.cuh:42-100,
+ * src/evaluators/evaluatorJ3DK.cuh:151-185)
+ * ==================================================================== */
+int i2_evaluator_create(i2_evaluator** out, const i2_mesh* m, i2_integrator* it);
+void i2_evaluator_destroy(i2_evaluator* ev);
+/* Evaluator3D::runAllPairs (src/evaluators/evaluator3d.cu:120-204): ordered
+ * task list per type = [(i,j,k) for the i<j list] ++ [(j,i,n+k)], integrated
+ * per type (vertex-sharing, edge-sharing, disjoint) with per-level launches;
+ * checkCorrectness computes the (i,j)/(j,i) symmetry error. */
+int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness);
+/* Evaluator3D::runPairs (src/evaluators/evaluator3d.cu:213-288): user task
+ * lists (i, j, k) per type; k is ignored (tasks are numbered by position). */
+int i2_evaluator_run_pairs(i2_evaluator* ev, const int32_t* simple3, int64_t nSimple,
+                           const int32_t* attached3, int64_t nAttached, const int32_t* not3, int64_t nNot);
+int i2_evaluator_task_count(const i2_evaluator* ev, int type, int64_t* n);
+/* Results of the last run for one type, in task order; any output may be NULL.
+ * tasks3: (i, j, k); integrals4: (Psi_x, Psi_y, Psi_z, Theta); results3: J;
+ * errors: symmetry error (only after runAllPairs with checkCorrectness);
+ * convLevel: refinement iteration at which the task converged (or the last). */
+int i2_evaluator_get_results(const i2_evaluator* ev, int type, int32_t* tasks3, double* integrals4,
+                             double* results3, double* errors, int8_t* convLevel);
+/* Number of refinement iterations K run for a type in the last run. */
+int i2_evaluator_refinement_iterations(const i2_evaluator* ev, int type, int* K);
+/* Device time (ms) of the last run for a type and the number of refined
+ * (sub-panel, panel) tasks integrated. */
+int i2_evaluator_type_stats(const i2_evaluator* ev, int type, double* ms, int64_t* refinedTasks);
+/* Evaluator3D::outputResultsToFile (src/evaluators/evaluator3d.cu:354-457):
+ * writes SimpleNeighbors/AttachedNeighbors/NotNeighbors .dat or .csv into dir
+ * (NULL = current directory). */
+int i2_evaluator_output_results_to_file(const i2_evaluator* ev, int type, int format, const char* dir);
+
+/* ======================================================================
+ * Synthetic meshes (no counterpart in the reference, which loads files)
+ * Call once with NULL buffers to get nV and nT.
+ * ==================================================================== */
+/* Geodesic icosphere subdivided `level` times, scaled to semi-axes
+ * (ax, ay, az), each vertex moved along the ellipsoid normal by
+ * N(0, (jitter_rel * mean edge length)^2) from mt19937_64(seed). */
+int i2_meshgen_icosphere(int level, double ax, double ay, double az, double jitter_rel, uint64_t seed,
+                         double* xyz, int32_t* tri, int64_t* nV, int64_t* nT);
+/* Torus with major radius R, minor radius r on an nmajor x nminor grid, each
+ * quad split along a fixed diagonal, jittered along the surface normal. */
+int i2_meshgen_torus(int nmajor, int nminor, double R, double r, double jitter_rel, uint64_t seed,
+                     double* xyz, int32_t* tri, int64_t* nV, int64_t* nT);
+/* Subdivision levels (uniform on {2,3,4,5}) and semi-axes (uniform on
+ * (0.5, 3.0)) of a batch of nmeshes icospheres, drawn from mt19937_64(seed). */
+int i2_meshgen_batch_spec(int nmeshes, uint64_t seed, int32_t* levels, double* axes3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INTEGRATOR2_MI355X_H */

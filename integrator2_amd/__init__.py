@@ -1,0 +1,262 @@
+"""Python binding of the MI355X panel-pair integrator (ctypes over the C ABI).
+
+This is plumbing for tests and benchmarks: every computation runs in the HIP
+library ``libintegrator2_mi355x.so`` built by ``integrator2_amd.build``.  The
+classes mirror the reference's C++ API (Mesh3D, NumericalIntegrator3D,
+EvaluatorJ3DK in andreyypopov/integrator2 src/) with the same names and
+argument meaning.  There is no fallback: if the library is missing or no GPU
+is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libintegrator2_mi355x.so")
+
+SIMPLE_NEIGHBORS = 0     # one shared vertex (vertex-sharing)
+ATTACHED_NEIGHBORS = 1   # shared edge (edge-sharing)
+NOT_NEIGHBORS = 2        # disjoint
+TYPE_NAMES = {0: "vertex-sharing", 1: "edge-sharing", 2: "disjoint"}
+
+PLAIN_TEXT = 1
+CSV = 2
+
+# Cowper's 13-point rule (7th order) with the reference's 15-digit literals,
+# src/QuadratureFormula3d.cuh:181-213 (qf3D13).
+QF3D13_L12 = np.array([
+    [0.333333333333333, 0.333333333333333],
+    [0.479308067841923, 0.260345966079038],
+    [0.260345966079038, 0.479308067841923],
+    [0.260345966079038, 0.260345966079038],
+    [0.869739794195598, 0.065130102902216],
+    [0.065130102902216, 0.869739794195598],
+    [0.065130102902216, 0.065130102902216],
+    [0.638444188569809, 0.312865496004875],
+    [0.312865496004875, 0.638444188569809],
+    [0.638444188569809, 0.048690315425316],
+    [0.048690315425316, 0.638444188569809],
+    [0.312865496004875, 0.048690315425316],
+    [0.048690315425316, 0.312865496004875],
+])
+QF3D13_W = np.array([-0.149570044467670] + [0.175615257433204] * 3 + [0.053347235608839] * 3 + [0.077113760890257] * 6)
+QF3D13_ORDER = 7
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_lib = None
+
+
+class I2Error(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise I2Error(f"{LIB_PATH} is missing: run integrator2_amd.build.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    L.i2_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(status):
+    if status != 0:
+        msg = lib().i2_last_error().decode(errors="replace")
+        raise I2Error(f"status {status}: {msg}")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    _check(lib().i2_device_count(ctypes.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------- meshes
+def icosphere(level, axes=(1.0, 1.0, 1.0), jitter=0.0, seed=0):
+    """Icosphere subdivided `level` times mapped to an ellipsoid, jittered along its normal."""
+    L = lib()
+    nV, nT = _I64(), _I64()
+    args = [ctypes.c_int(level), _D(axes[0]), _D(axes[1]), _D(axes[2]), _D(jitter), ctypes.c_uint64(seed)]
+    _check(L.i2_meshgen_icosphere(*args, None, None, ctypes.byref(nV), ctypes.byref(nT)))
+    xyz = np.zeros((nV.value, 3))
+    tri = np.zeros((nT.value, 3), np.int32)
+    _check(L.i2_meshgen_icosphere(*args, _ptr(xyz), _ptr(tri), ctypes.byref(nV), ctypes.byref(nT)))
+    return xyz, tri
+
+
+def torus(nmajor=256, nminor=128, R=1.0, r=0.25, jitter=0.0, seed=0):
+    L = lib()
+    nV, nT = _I64(), _I64()
+    args = [ctypes.c_int(nmajor), ctypes.c_int(nminor), _D(R), _D(r), _D(jitter), ctypes.c_uint64(seed)]
+    _check(L.i2_meshgen_torus(*args, None, None, ctypes.byref(nV), ctypes.byref(nT)))
+    xyz = np.zeros((nV.value, 3))
+    tri = np.zeros((nT.value, 3), np.int32)
+    _check(L.i2_meshgen_torus(*args, _ptr(xyz), _ptr(tri), ctypes.byref(nV), ctypes.byref(nT)))
+    return xyz, tri
+
+
+def batch_spec(nmeshes=64, seed=0):
+    levels = np.zeros(nmeshes, np.int32)
+    axes = np.zeros((nmeshes, 3))
+    _check(lib().i2_meshgen_batch_spec(ctypes.c_int(nmeshes), ctypes.c_uint64(seed), _ptr(levels), _ptr(axes)))
+    return levels, axes
+
+
+# ---------------------------------------------------------------- Mesh3D
+class Mesh3D:
+    """Mesh3D (src/Mesh3d.cuh:55-126): vertices, cells, normals, areas, neighbour lists."""
+
+    def __init__(self, device=0):
+        self._h = _P()
+        _check(lib().i2_mesh_create(ctypes.byref(self._h), ctypes.c_int(device)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.i2_mesh_destroy(self._h)
+            self._h = None
+
+    def loadMeshFromFile(self, filename, scale=1.0):
+        st = lib().i2_mesh_load_file(self._h, filename.encode(), _D(scale))
+        if st != 0:
+            return False
+        return True
+
+    def setData(self, xyz, tri):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        tri = np.ascontiguousarray(tri, np.int32)
+        _check(lib().i2_mesh_set_data(self._h, _ptr(xyz), _I64(len(xyz)), _ptr(tri), _I64(len(tri))))
+
+    def prepareMesh(self):
+        _check(lib().i2_mesh_prepare(self._h))
+
+    def sizes(self):
+        nV, nT = _I64(), _I64()
+        pairs = (ctypes.c_int64 * 3)()
+        _check(lib().i2_mesh_sizes(self._h, ctypes.byref(nV), ctypes.byref(nT), pairs))
+        return nV.value, nT.value, [pairs[0], pairs[1], pairs[2]]
+
+    def getVertices(self):
+        out = np.zeros((self.sizes()[0], 3))
+        _check(lib().i2_mesh_copy_vertices(self._h, _ptr(out)))
+        return out
+
+    def getCells(self):
+        out = np.zeros((self.sizes()[1], 3), np.int32)
+        _check(lib().i2_mesh_copy_cells(self._h, _ptr(out)))
+        return out
+
+    def getCellNormals(self):
+        out = np.zeros((self.sizes()[1], 3))
+        _check(lib().i2_mesh_copy_normals(self._h, _ptr(out)))
+        return out
+
+    def getCellMeasures(self):
+        out = np.zeros(self.sizes()[1])
+        _check(lib().i2_mesh_copy_measures(self._h, _ptr(out)))
+        return out
+
+    def _neighbors(self, t):
+        n = self.sizes()[2][t]
+        out = np.zeros((n, 3), np.int32)
+        _check(lib().i2_mesh_copy_neighbors(self._h, ctypes.c_int(t), _ptr(out)))
+        return out
+
+    def getSimpleNeighbors(self):
+        return self._neighbors(SIMPLE_NEIGHBORS)
+
+    def getAttachedNeighbors(self):
+        return self._neighbors(ATTACHED_NEIGHBORS)
+
+    def getNotNeighbors(self):
+        return self._neighbors(NOT_NEIGHBORS)
+
+
+# ---------------------------------------------------------------- NumericalIntegrator3D
+class NumericalIntegrator3D:
+    """NumericalIntegrator3D(mesh, qf) (src/NumericalIntegrator3d.cuh:89-327)."""
+
+    def __init__(self, mesh, L12=QF3D13_L12, weights=QF3D13_W, order=QF3D13_ORDER):
+        self.mesh = mesh
+        self._L12 = np.ascontiguousarray(L12, np.float64)
+        self._w = np.ascontiguousarray(weights, np.float64)
+        self._h = _P()
+        _check(lib().i2_integrator_create(ctypes.byref(self._h), mesh._h, _ptr(self._L12), _ptr(self._w),
+                                          ctypes.c_int(len(self._w)), ctypes.c_int(order)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.i2_integrator_destroy(self._h)
+            self._h = None
+
+    def setFixedRefinementLevel(self, refinementLevel=0):
+        _check(lib().i2_integrator_set_fixed_refinement_level(self._h, ctypes.c_int(refinementLevel)))
+
+    def setErrorControl(self, maxLevel=5, tol=1e-5, refParityQuirk=True):
+        _check(lib().i2_integrator_set_error_control(self._h, ctypes.c_int(maxLevel), _D(tol), ctypes.c_int(int(refParityQuirk))))
+
+
+# ---------------------------------------------------------------- EvaluatorJ3DK
+class EvaluatorJ3DK:
+    """EvaluatorJ3DK(mesh, integrator) (src/evaluators/evaluatorJ3DK.cuh:151-185)."""
+
+    def __init__(self, mesh, integrator):
+        self.mesh, self.integrator = mesh, integrator
+        self._h = _P()
+        _check(lib().i2_evaluator_create(ctypes.byref(self._h), mesh._h, integrator._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.i2_evaluator_destroy(self._h)
+            self._h = None
+
+    def runAllPairs(self, checkCorrectness=False):
+        _check(lib().i2_evaluator_run_all_pairs(self._h, ctypes.c_int(int(checkCorrectness))))
+
+    def runPairs(self, simple, attached, notn):
+        arrs = [np.ascontiguousarray(np.asarray(a, np.int32).reshape(-1, 3)) for a in (simple, attached, notn)]
+        _check(lib().i2_evaluator_run_pairs(self._h, _ptr(arrs[0]), _I64(len(arrs[0])), _ptr(arrs[1]), _I64(len(arrs[1])),
+                                            _ptr(arrs[2]), _I64(len(arrs[2]))))
+
+    def taskCount(self, t):
+        n = _I64()
+        _check(lib().i2_evaluator_task_count(self._h, ctypes.c_int(t), ctypes.byref(n)))
+        return n.value
+
+    def results(self, t, errors=False):
+        """Dict with tasks (n,3), integrals (n,4), results (n,3), conv (n,), errors (n,) if requested."""
+        n = self.taskCount(t)
+        out = dict(tasks=np.zeros((n, 3), np.int32), integrals=np.zeros((n, 4)), results=np.zeros((n, 3)),
+                   conv=np.zeros(n, np.int8))
+        err = np.zeros(n) if errors else None
+        _check(lib().i2_evaluator_get_results(self._h, ctypes.c_int(t), _ptr(out["tasks"]), _ptr(out["integrals"]),
+                                              _ptr(out["results"]), _ptr(err), _ptr(out["conv"])))
+        if errors:
+            out["errors"] = err
+        return out
+
+    def refinementIterations(self, t):
+        K = ctypes.c_int()
+        _check(lib().i2_evaluator_refinement_iterations(self._h, ctypes.c_int(t), ctypes.byref(K)))
+        return K.value
+
+    def typeStats(self, t):
+        ms = _D()
+        n = _I64()
+        _check(lib().i2_evaluator_type_stats(self._h, ctypes.c_int(t), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def outputResultsToFile(self, t, fmt, directory=None):
+        st = lib().i2_evaluator_output_results_to_file(self._h, ctypes.c_int(t), ctypes.c_int(fmt),
+                                                       None if directory is None else directory.encode())
+        return st == 0

@@ -1,0 +1,444 @@
+// i2_device.hpp -- gfx950 device math of the panel-pair integrator (FP64).
+//
+// Formulas follow andreyypopov/integrator2 src/evaluators/evaluatorJ3DK.cu
+// (cited per function); the layout is MI355X-first: per-panel invariants
+// (edge tangents, normal, area) live in a 192-byte Panel record built once
+// per mesh, per-pair invariants of the singular parts are built once per task
+// and kept in registers, and only the per-point work runs inside the
+// quadrature loops.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace i2d {
+
+// ------------------------------------------------------------ constants
+// src/common/constants.h:13-24
+constexpr double DOUBLE_MIN = 2e-6;
+constexpr double EPS_ZERO = 1e-6;
+constexpr double EPS_ZERO2 = 1e-10;
+constexpr double EPS_PSI_THETA2 = 1e-6 * 1e-6;
+constexpr double PI = 3.14159265358979323846;
+constexpr double TWO_PI = 6.28318530717958647692;
+constexpr double RECIP_4PI = 0.079577471545947667884;
+constexpr double LOG2 = 0.69314718055994530942;
+
+constexpr int MAX_POINTS = 16;   // largest rule has 13 points
+
+// Quadrature rule passed by value as a kernel argument (lands in SGPRs):
+// barycentric (L1, L2, L3 = 1 - L1 - L2) and weights, per
+// src/NumericalIntegrator3d.cu:197-216.
+struct Rule {
+    double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
+    double pow2p;   // 2^order, the Runge factor (src/evaluators/evaluator3d.cu:101-106)
+    int n;
+    int order;
+};
+
+// ------------------------------------------------------------ vectors
+struct d3 { double x, y, z; };
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ d3 operator*(double s, d3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ double vlen(d3 a) { return sqrt(dot(a, a)); }
+__device__ __forceinline__ d3 unit(d3 a) { const double inv = 1.0 / vlen(a); return {a.x * inv, a.y * inv, a.z * inv}; }
+__device__ __forceinline__ d3 divs(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+
+// (Psi, Theta) accumulator
+struct q4 { double x, y, z, w; };
+__device__ __forceinline__ q4 q4zero() { return {0.0, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ q4 operator+(q4 a, q4 b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+__device__ __forceinline__ q4 operator-(q4 a, q4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+__device__ __forceinline__ q4 operator*(double s, q4 a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+
+// Per-panel record: vertices A, B, C (reference vertex order), the edge
+// tangents of the inner integral (taua = (C-B)/|C-B|, taub = (A-C)/|A-C|,
+// tauc = (B-A)/|B-A|, evaluatorJ3DK.cu:286-288), unit normal and area
+// (src/Mesh3d.cu:23-32, 62-71) and the vertex indices.
+struct alignas(16) Panel {
+    d3 a, b, c;
+    d3 ta, tb, tc;
+    d3 n;
+    double area;
+    int v0, v1, v2, pad;
+};
+static_assert(sizeof(Panel) == 192, "Panel record is 192 bytes");
+
+// ------------------------------------------------------------ helpers of cuda_math
+// src/common/cuda_math.cuh:31-49
+__device__ __forceinline__ double sign_db(double x) {
+    if (fabs(x) < DOUBLE_MIN) return 0.0;
+    return (x > DOUBLE_MIN) ? 1.0 : -1.0;
+}
+__device__ __forceinline__ double arg_db(double x) { return (x > DOUBLE_MIN) ? 0.0 : PI; }
+// src/common/cuda_math.cu:14-27
+__device__ __forceinline__ double angle(d3 a, d3 b) {
+    const double den = sqrt(dot(a, a) * dot(b, b));
+    if (den < EPS_ZERO) return 0.0;
+    const double r = dot(a, b) / den;
+    if (r >= 1.0) return 0.0;
+    if (r <= -1.0) return PI;
+    return acos(r);
+}
+
+// ------------------------------------------------------------ inner integral
+// evaluatorJ3DK.cu:266-313.  (Psi, Theta) of panel J at point p, with
+// Theta n_J + Psi x n_J = integral over K_J of (p - r)/|p - r|^3.
+__device__ __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, const d3& ta, const d3& tb, const d3& tc) {
+    d3 ova = p - A, ovb = p - B, ovc = p - C;
+    const double lva = vlen(ova), lvb = vlen(ovb), lvc = vlen(ovc);
+    ova = divs(ova, lva); ovb = divs(ovb, lvb); ovc = divs(ovc, lvc);
+    const double rac = dot(ova, tc), rbc = dot(ovb, tc), rba = dot(ovb, ta),
+                 rca = dot(ovc, ta), rcb = dot(ovc, tb), rab = dot(ova, tb);
+    // the degenerate branches (point on an edge line) keep the ratio of distances
+    const double n1 = (fabs(rbc + 1.0) < 0.5 * EPS_PSI_THETA2) ? lvb : lva * (1.0 + rac);
+    const double d1 = (fabs(rbc + 1.0) < 0.5 * EPS_PSI_THETA2) ? lva : lvb * (1.0 + rbc);
+    const double n2 = (fabs(rca + 1.0) < 0.5 * EPS_PSI_THETA2) ? lvc : lvb * (1.0 + rba);
+    const double d2 = (fabs(rca + 1.0) < 0.5 * EPS_PSI_THETA2) ? lvb : lvc * (1.0 + rca);
+    const double n3 = (fabs(rab + 1.0) < 0.5 * EPS_PSI_THETA2) ? lva : lvc * (1.0 + rcb);
+    const double d3_ = (fabs(rab + 1.0) < 0.5 * EPS_PSI_THETA2) ? lvc : lva * (1.0 + rab);
+    const double t1 = log(n1 / d1), t2 = log(n2 / d2), t3 = log(n3 / d3_);
+    q4 r;
+    r.x = t1 * tc.x + t2 * ta.x + t3 * tb.x;
+    r.y = t1 * tc.y + t2 * ta.y + t3 * tb.y;
+    r.z = t1 * tc.z + t2 * ta.z + t3 * tb.z;
+    r.w = 2.0 * atan2(dot(cross(ova, ovb), ovc), 1.0 + dot(ova, ovb) + dot(ovb, ovc) + dot(ovc, ova));
+    return r;
+}
+
+// ------------------------------------------------------------ singular parts
+// Edge-sharing pair: vertices of K_j rotated so that JA is off the shared
+// edge (evaluatorJ3DK.cu:797-847), plus the tangents and 1/|JC-JB| that
+// evaluatorJ3DK.cu:338-343 recompute per point.
+struct EdgeCtx { d3 JB, JC, taua, taub, tauc; double ilvc; };
+__device__ __forceinline__ EdgeCtx edge_ctx(d3 JA, d3 JB, d3 JC) {
+    EdgeCtx c;
+    c.JB = JB; c.JC = JC;
+    c.taua = unit(JA - JC);
+    c.taub = unit(JB - JA);
+    d3 tc = JC - JB;
+    c.ilvc = 1.0 / vlen(tc);
+    c.tauc = c.ilvc * tc;
+    return c;
+}
+// evaluatorJ3DK.cu:315-350
+__device__ __forceinline__ q4 sing_edge(d3 p, const EdgeCtx& c) {
+    d3 va = p - c.JB, vb = p - c.JC;
+    const double lva = vlen(va), lvb = vlen(vb);
+    va = divs(va, lva); vb = divs(vb, lvb);
+    const double l1 = log((lvb * dot(c.tauc, c.tauc - vb)) / (lva * dot(c.tauc, c.tauc - va)));
+    const double l2 = log(lva * dot(c.taub, c.taub + va) * c.ilvc);
+    const double l3 = log(lvb * dot(c.taua, c.taua - vb) * c.ilvc);
+    q4 r;
+    r.x = l1 * c.tauc.x - l2 * c.taub.x - l3 * c.taua.x;
+    r.y = l1 * c.tauc.y - l2 * c.taub.y - l3 * c.taua.y;
+    r.z = l1 * c.tauc.z - l2 * c.taub.z - l3 * c.taua.z;
+    r.w = 2.0 * (atan2(dot(cross(va, c.taub), c.tauc), dot(c.taub - c.tauc, c.taub + va)) -
+                 atan2(dot(cross(vb, c.taua), c.tauc), dot(c.taua - c.tauc, c.taua - vb)));
+    return r;
+}
+
+// Vertex-sharing pair: JA is the shared vertex; e and the two delta angles
+// follow evaluatorJ3DK.cu:376-399 (two sign flips of e).
+struct VertexCtx { d3 JA, taua, taub, e; double invAri; };
+__device__ __forceinline__ d3 vertex_e(d3 nI, d3 nJ, d3 taua, d3 taub, double* dA, double* dB) {
+    d3 e = cross(nI, nJ);
+    if (dot(e, e) < EPS_ZERO2) e = taub;
+    else e = unit(e);
+    double dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
+    double dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
+    if ((PI - fabs(dx) < EPS_ZERO) || (PI - fabs(dy) < EPS_ZERO)) {
+        e = -e;
+        dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
+        dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
+    }
+    if ((dx * dy < 0) && (fabs(dx - dy) > PI)) {
+        e = -e;
+        dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
+        dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
+    }
+    *dA = dx; *dB = dy;
+    return e;
+}
+__device__ __forceinline__ VertexCtx vertex_ctx(d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
+    VertexCtx c;
+    c.JA = JA;
+    c.taua = unit(JA - JC);
+    c.taub = unit(JB - JA);
+    double dA, dB;
+    c.e = vertex_e(nI, nJ, c.taua, c.taub, &dA, &dB);
+    c.invAri = 1.0 / sqrt(areaI);   // rsqrt(measures[i]), evaluatorJ3DK.cu:401
+    return c;
+}
+// evaluatorJ3DK.cu:352-407
+__device__ __forceinline__ q4 sing_vertex(d3 p, const VertexCtx& c) {
+    d3 ovc = p - c.JA;
+    const double lvc = vlen(ovc);
+    ovc = divs(ovc, lvc);
+    const double la = log((lvc * (1.0 + dot(c.taua, ovc))) * c.invAri);
+    const double lb = log((lvc * (1.0 - dot(c.taub, ovc))) * c.invAri);
+    q4 r;
+    r.x = -(la * c.taua.x + lb * c.taub.x);
+    r.y = -(la * c.taua.y + lb * c.taub.y);
+    r.z = -(la * c.taua.z + lb * c.taub.z);
+    r.w = 2.0 * (atan2(dot(cross(ovc, c.taua), c.e), dot(c.e - ovc, c.e - c.taua)) +
+                 atan2(dot(cross(ovc, c.taub), c.e), dot(c.e - ovc, c.e + c.taub)));
+    return r;
+}
+
+// ------------------------------------------------------------ topology of a pair
+__device__ __forceinline__ int vid(const Panel& P, int k) { return k == 0 ? P.v0 : (k == 1 ? P.v1 : P.v2); }
+__device__ __forceinline__ d3 vtx(const Panel& P, int k) { return k == 0 ? P.a : (k == 1 ? P.b : P.c); }
+// number of shared vertex indices (src/Mesh3d.cu:111-122)
+__device__ __forceinline__ int shared_count(const Panel& I, const Panel& J) {
+    int c = 0;
+    if (I.v0 == J.v0 || I.v0 == J.v1 || I.v0 == J.v2) ++c;
+    if (I.v1 == J.v0 || I.v1 == J.v1 || I.v1 == J.v2) ++c;
+    if (I.v2 == J.v0 || I.v2 == J.v1 || I.v2 == J.v2) ++c;
+    return c;
+}
+// evaluatorJ3DK.cu:782-795: position of the shared vertex (last match wins)
+__device__ __forceinline__ void shifts_vertex(const Panel& I, const Panel& J, int* s1, int* s2) {
+    int a = 0, b = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (vid(I, i) == vid(J, j)) { a = i; b = j; }
+    *s1 = a; *s2 = b;
+}
+// evaluatorJ3DK.cu:797-825: position of the vertex off the shared edge
+__device__ __forceinline__ void shifts_edge(const Panel& I, const Panel& J, int* s1, int* s2) {
+    int mi = 0, mj = 0;   // bit masks of matched positions
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (vid(I, i) == vid(J, j)) { mi |= 1 << i; mj |= 1 << j; }
+    *s1 = (mi & 1) == 0 ? 0 : ((mi & 2) == 0 ? 1 : 2);
+    *s2 = (mj & 1) == 0 ? 0 : ((mj & 2) == 0 ? 1 : 2);
+}
+
+// ------------------------------------------------------------ analytic singular integrals
+// evaluatorJ3DK.cu:418-421
+__device__ __forceinline__ double phi_f(double sA, double cA, double sG, double cG, double sXi, double cosLambda) {
+    return 2.0 * atan2(sXi * sA * sG, 1.0 - cA + cG + cosLambda);
+}
+// evaluatorJ3DK.cu:438-461 -> (q_Theta, q_Psi)
+__device__ inline void q_theta_psi(double sA, double cA, double sB, double cB, double sG, double cG, double sNu, double cNu,
+                                   double sXi, double cXi, double cosMu, double cosLambda, double* qT, double* qP) {
+    const double phi1 = phi_f(sA, cA, sG, cG, sXi, cosLambda);
+    const double phi2 = phi_f(sA, -cA, sG, -cG, sXi, cosLambda);
+    const double iS = 1.0 / (sA * (1.0 - cosMu * cosMu));
+    *qT = phi1 + sG * sNu * iS * ((cB * sG - cXi * sB * cG) * phi2 +
+                                  sXi * sB * (0.5 * (1.0 + cosMu) * log((1.0 + cB) / (1.0 - cNu)) +
+                                              0.5 * (1.0 - cosMu) * log((1.0 - cB) / (1.0 + cNu)) +
+                                              log((1.0 + cosLambda) / (1.0 - cG))));
+    *qP = 1.5 - iS * (sB * (cNu + cosMu * cosLambda) * log(1.0 + cosLambda) +
+                      sNu * (cB + cosMu * cG) * log(1.0 - cG) +
+                      sB * (1.0 - cosMu) * (cNu - cosLambda) * log(sB / sNu) +
+                      sNu * sB * (sB * cG - cXi * sG * cB) * log((1.0 - cNu) / (1.0 + cB)) +
+                      phi2 * sXi * sG * sNu * sB);
+}
+// evaluatorJ3DK.cu:474-481 (coplanar case; q_Theta takes its documented value 0)
+__device__ inline void q_theta_psi_zero(double sB, double cB, double sNu, double cNu, double sinAlpha, double* qT, double* qP) {
+    *qT = 0.0;
+    *qP = 1.5 - (cNu * sB * log(1.0 + cNu) + sNu * cB * log(1.0 - cB) + sinAlpha - sB + sNu + sB * cNu * log(sB / sNu)) / sinAlpha;
+}
+
+// evaluatorJ3DK.cu:603-672: closed-form integral over K_i of the edge-sharing singular part.
+// (IA, IB, IC) and (JA, JB, JC) are rotated so that IA and JA are off the shared edge.
+__device__ inline q4 integrate_sing_edge(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
+    const d3 taua = unit(JA - JC), taub = unit(JB - JA), tauc = unit(JC - JB);
+    const double alpha = angle(IA - IC, IB - IC);
+    const double beta = angle(IC - IB, IA - IB);
+    const double gamma = angle(JC - JB, JA - JB);
+    const double delta = angle(JB - JC, JA - JC);
+    const double nu = PI - alpha - beta;
+    const double xi = atan2(dot(cross(nI, nJ), tauc), dot(nI, nJ));
+    double sA, cA, sB, cB, sG, cG, sD, cD, sX, cX;
+    sincos(alpha, &sA, &cA); sincos(beta, &sB, &cB); sincos(gamma, &sG, &cG);
+    sincos(delta, &sD, &cD); sincos(xi, &sX, &cX);
+    const double sNu = sA * cB + cA * sB;
+    const double cNu = sA * sB - cA * cB;
+    const double cosSigma = -(cA * cD + cX * sA * sD);
+    const double cosMu = -(cB * cG + cX * sB * sG);
+    const double cosLambda = -(cA * cG - cX * sA * sG);
+    const double cosTheta = -(cB * cD - cX * sB * sD);
+    const double qab = sNu * log(tan(0.5 * alpha) * tan(0.5 * nu)) / sB +
+                       sNu * log(tan(0.5 * beta) * tan(0.5 * nu)) / sA +
+                       log(tan(0.5 * alpha) * tan(0.5 * beta));
+    double qAT, qAP, qBT, qBP;
+    if ((fabs(xi) < EPS_ZERO) && (fabs(beta - gamma) < EPS_ZERO)) q_theta_psi_zero(sB, cB, sNu, cNu, sA, &qAT, &qAP);
+    else q_theta_psi(sA, cA, sB, cB, sG, cG, sNu, cNu, sX, cX, cosMu, cosLambda, &qAT, &qAP);
+    if ((fabs(xi) < EPS_ZERO) && (fabs(alpha - delta) < EPS_ZERO)) q_theta_psi_zero(sA, cA, sNu, cNu, sB, &qBT, &qBP);
+    else q_theta_psi(sB, cB, sA, cA, sD, cD, sNu, cNu, sX, cX, cosSigma, cosTheta, &qBT, &qBP);
+    q4 r;
+    const d3 v = areaI * (qAP * taub + qBP * taua - qab * tauc);
+    r.x = v.x; r.y = v.y; r.z = v.z;
+    r.w = areaI * (qAT + qBT);
+    return r;
+}
+
+// evaluatorJ3DK.cu:511-601 -> (q^Theta, q^Psi) for vertex-sharing pairs
+__device__ inline void q_theta_psi_cont(double sXi, double cXi, double mu, double sMu, double cMu, double logSinMu, double logSinNu,
+                                        double psi, double sPsi, double cPsi, double nu, double sNu, double cNu, double kappa,
+                                        double sinKappa, double sinNuPsi, double sinMuPsi, double delta, double sD, double cD,
+                                        double cosLambda, double cosTheta, double cosEta, double cosSigma, double cosChi,
+                                        double* qT, double* qP) {
+    const double logOneCosTheta = log(1.0 + cosTheta);
+    const double logOneCosLambda = log(1.0 + cosLambda);
+    const double Lambda1 = logOneCosLambda - logOneCosTheta + logSinNu - logSinMu;
+    const double Lambda2 = log(tan(0.5 * nu) * tan(0.5 * mu));
+    double s05D, c05D;
+    sincos(0.5 * delta, &s05D, &c05D);
+    const double tan05Delta = s05D / c05D;
+    double sMP, cMP, sNP, cNP;
+    sincos(0.5 * (mu - psi), &sMP, &cMP);
+    sincos(0.5 * (nu + psi), &sNP, &cNP);
+    const double Amu = atan2(tan05Delta * cMP * sXi, tan05Delta * cMP * cXi + sMP);
+    const double Anu = atan2(tan05Delta * sNP * sXi, tan05Delta * sNP * cXi + cNP);
+    double sT, cT, sT2, cT2;
+    sincos(0.5 * (mu - psi) - 0.5 * (nu + psi), &sT, &cT);
+    sincos(0.5 * kappa, &sT2, &cT2);
+    const double W = atan2(sD * sT2 * sXi, cT2 + sD * cT * cXi + cD * sT);
+    sincos(delta - psi, &sT, &cT);
+    const double D = 1.0 / (sT * sT + sD * sPsi * (1.0 - cXi) * (cT + cosSigma));
+    const double G = cPsi * (sD * cosSigma * cXi + cD * cosChi - sD * sD / sPsi);
+    const double gent = 2.0 * (Anu * sMu * sinNuPsi - Amu * sNu * sinMuPsi -
+                               D * sMu * sNu * sD * (W * cosEta + 0.5 * sPsi * sXi * (Lambda1 - Lambda2 * cosSigma))) /
+                        (sPsi * sinKappa);
+    const double gens = 0.5 * (3.0 - LOG2) +
+                        (sMu * sNu / sinKappa) * ((logOneCosLambda - logOneCosTheta) * cPsi / sPsi +
+                                                  D * (Lambda1 * sD * cosEta / sPsi + Lambda2 * cosChi -
+                                                       2.0 * W * sD * sXi - G * (logSinNu - logSinMu))) -
+                        (cMu * sNu * (logOneCosLambda - logSinMu) + sMu * cNu * (logOneCosTheta - logSinNu)) / sinKappa -
+                        0.5 * (logSinMu + logSinNu - log(sinKappa));
+    const double mulPsi = sign_db(sXi * sPsi);
+    const double mulDelta = sign_db(sXi * sD);
+    if ((fabs(sXi) < EPS_ZERO) && (1.0 - fabs(cosSigma) < 0.5 * EPS_ZERO2) && (fabs(sPsi) > EPS_ZERO)) {
+        const double ara = arg_db(sin(0.5 * (nu + psi)) * cosSigma);
+        const double arb = arg_db(cos(0.5 * (mu - psi)) * cosSigma);
+        *qT = 2.0 * mulPsi * cosSigma * cXi / (sinKappa * sPsi) * (sMu * sinNuPsi * ara - sNu * sinMuPsi * arb);
+        *qP = 0.5 * (1.0 - LOG2) - 0.5 * log((1.0 - cosSigma * cMu) * (1.0 + cosSigma * cNu) / sinKappa) +
+              cosSigma * (sMu - sNu + 0.5 * sin(mu - nu) * Lambda2) / sinKappa;
+        return;
+    }
+    if ((fabs(sXi) < EPS_ZERO) && (fabs(sPsi) > EPS_ZERO)) {
+        const double are = arg_db(sin(0.5 * (nu + mu)) + sin(0.5 * (mu - nu) - psi + delta * cXi));
+        const double arc = arg_db(1.0 / tan(0.5 * (nu + psi)) + tan(0.5 * delta) * cXi);
+        const double ard = arg_db(tan(0.5 * (mu - psi)) + tan(0.5 * delta) * cXi);
+        *qT = 2.0 * mulDelta * (sD * sMu * sNu / cosChi * cXi * are + sMu * sinNuPsi * arc - sNu * sinMuPsi * ard) / (sinKappa * sPsi);
+        *qP = gens;
+        return;
+    }
+    if (fabs(sin(psi)) < EPS_ZERO) {
+        if (fabs(delta) > EPS_ZERO) {
+            *qT = 2.0 * (sMu * sNu * (W * cD * cXi - 0.5 * (Lambda1 - Lambda2 * cosSigma) * sXi) / sD +
+                         (Anu + mulDelta * arg_db(sin(0.5 * (nu + psi)))) * sMu * cNu +
+                         (Amu + mulDelta * arg_db(cos(0.5 * (mu - psi)))) * cMu * sNu) / sinKappa;
+            *qP = 0.5 * (3.0 - LOG2) - 0.5 * (log((1.0 + cosLambda) * (1.0 + cosTheta) / sinKappa) - sin(mu - nu) / sinKappa * Lambda1) -
+                  sMu * sNu * ((Lambda1 * cD - Lambda2 * cPsi) * cXi + 2.0 * W * sXi) / (sinKappa * sD);
+        } else {
+            *qT = 2.0 * arg_db(cPsi);
+            *qP = 0.5 * (1.0 - LOG2) - 0.5 * log((1.0 - cPsi * cMu) * (1.0 + cPsi * cNu) / sinKappa) +
+                  (0.5 * sin(mu - nu) * Lambda1 + cPsi * (sMu - sNu)) / sinKappa;
+        }
+        return;
+    }
+    *qT = gent;
+    *qP = gens;
+}
+
+// evaluatorJ3DK.cu:674-780: closed-form integral over K_i of the vertex-sharing
+// singular part.  IA and JA are the shared vertex.
+__device__ inline q4 integrate_sing_vertex(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
+    const d3 taua = unit(JA - JC), taub = unit(JB - JA);
+    double dA, dB;
+    const d3 e = vertex_e(nI, nJ, taua, taub, &dA, &dB);
+    const double xi = atan2(dot(cross(nI, nJ), e), dot(nI, nJ));
+    double sDA, cDA, sDB, cDB, sX, cX;
+    sincos(dA, &sDA, &cDA); sincos(dB, &sDB, &cDB); sincos(xi, &sX, &cX);
+    const d3 s = IC - IB;
+    const double nu = angle(IA - IB, IC - IB);
+    const double mu = angle(IB - IC, IA - IC);
+    const double kappa = angle(IB - IA, IC - IA);
+    double sMu, cMu, sNu, cNu;
+    sincos(mu, &sMu, &cMu); sincos(nu, &sNu, &cNu);
+    const double logSinNu = log(sNu), logSinMu = log(sMu);
+    const double sinKappa = sin(kappa);
+    const double psi = atan2(dot(cross(e, s), nI), dot(e, s));
+    double sPsi, cPsi, sNP, cNP, sMP, cMP;
+    sincos(psi, &sPsi, &cPsi);
+    sincos(nu + psi, &sNP, &cNP);
+    sincos(mu - psi, &sMP, &cMP);
+    double qAT, qAP, qBT, qBP;
+    {
+        const double cosSigma = sDA * sPsi * cX + cDA * cPsi;
+        const double cosChi = sDA * cPsi * cX - cDA * sPsi;
+        const double cosEta = cDA * sPsi * cX - sDA * cPsi;
+        const double cosTheta = sDA * sNP * cX + cDA * cNP;
+        const double cosLambda = sDA * sMP * cX - cDA * cMP;
+        q_theta_psi_cont(sX, cX, mu, sMu, cMu, logSinMu, logSinNu, psi, sPsi, cPsi, nu, sNu, cNu, kappa, sinKappa,
+                         sNP, sMP, dA, sDA, cDA, cosLambda, cosTheta, cosEta, cosSigma, cosChi, &qAT, &qAP);
+    }
+    {
+        const double cosSigma = sDB * sPsi * cX + cDB * cPsi;
+        const double cosChi = sDB * cPsi * cX - cDB * sPsi;
+        const double cosEta = cDB * sPsi * cX - sDB * cPsi;
+        const double cosTheta = sDB * sNP * cX + cDB * cNP;
+        const double cosLambda = sDB * sMP * cX - cDB * cMP;
+        q_theta_psi_cont(sX, cX, mu, sMu, cMu, logSinMu, logSinNu, psi, sPsi, cPsi, nu, sNu, cNu, kappa, sinKappa,
+                         sNP, sMP, dB, sDB, cDB, cosLambda, cosTheta, cosEta, cosSigma, cosChi, &qBT, &qBP);
+    }
+    q4 r;
+    const d3 v = areaI * (qAP * taua + qBP * taub);
+    r.x = v.x; r.y = v.y; r.z = v.z;
+    r.w = (fabs(xi) < EPS_ZERO) ? 0.0 : (areaI * (qAT - qBT));
+    return r;
+}
+
+// ------------------------------------------------------------ finalize
+// evaluatorJ3DK.cu:224-241 (vertex-sharing, with the 2*pi*p wrap of Theta)
+__device__ __forceinline__ d3 finalize_vertex(q4 I, d3 nJ, double areaI) {
+    int p = 0;
+    const double refTheta = TWO_PI * areaI;
+    if (I.w > refTheta) p = -((int)trunc((I.w - refTheta) / (2.0 * refTheta)) + 1);
+    else if (I.w < -refTheta) p = ((int)trunc((-refTheta - I.w) / (2.0 * refTheta)) + 1);
+    const d3 psi = {I.x, I.y, I.z};
+    return RECIP_4PI * ((I.w + 2.0 * p * refTheta) * nJ + cross(psi, nJ));
+}
+// evaluatorJ3DK.cu:254-264
+__device__ __forceinline__ d3 finalize_plain(q4 I, d3 nJ) {
+    const d3 psi = {I.x, I.y, I.z};
+    return RECIP_4PI * (I.w * nJ + cross(psi, nJ));
+}
+
+// ------------------------------------------------------------ Runge rule
+// src/common/cuda_math.cuh:270-300 and src/evaluators/evaluator3d.cu:83-92
+__device__ __forceinline__ double runge_criterion(q4 cur, q4 prev, double pow2p) {
+    const q4 num = cur - prev;
+    const q4 den = pow2p * prev - cur;
+    const double x = (fabs(num.x) < DOUBLE_MIN && fabs(den.x) < DOUBLE_MIN) ? 0.0 : num.x / den.x;
+    const double y = (fabs(num.y) < DOUBLE_MIN && fabs(den.y) < DOUBLE_MIN) ? 0.0 : num.y / den.y;
+    const double z = (fabs(num.z) < DOUBLE_MIN && fabs(den.z) < DOUBLE_MIN) ? 0.0 : num.z / den.z;
+    const double w = (fabs(num.w) < EPS_ZERO2 && fabs(den.w) < EPS_ZERO2) ? 0.0 : num.w / den.w;
+    return fabs(x) + fabs(y) + fabs(z) + fabs(w);
+}
+
+// ------------------------------------------------------------ sub-panels
+// A sub-panel of refinement level L is reached from the original panel by L
+// midpoint splits (src/NumericalIntegrator3d.cu:54-72): children
+// {m_AB, B, m_BC}, {m_BC, C, m_CA}, {m_CA, A, m_AB}, {m_BC, m_CA, m_AB}.
+// `path` holds the child digits, most significant split first.
+__device__ __forceinline__ void sub_panel(d3& a, d3& b, d3& c, int level, uint32_t path) {
+    for (int l = level - 1; l >= 0; --l) {
+        const int ch = (path >> (2 * l)) & 3;
+        const d3 mBC = 0.5 * (b + c), mCA = 0.5 * (c + a), mAB = 0.5 * (a + b);
+        if (ch == 0) { a = mAB; c = mBC; }
+        else if (ch == 1) { a = mBC; b = c; c = mCA; }
+        else if (ch == 2) { b = a; a = mCA; c = mAB; }
+        else { a = mBC; b = mCA; c = mAB; }
+    }
+}
+
+}  // namespace i2d

@@ -1,0 +1,563 @@
+// i2_evaluator.hip -- NumericalIntegrator3D + Evaluator3D/EvaluatorJ3DK behind
+// the C ABI: the "plain" path with per-type, per-level launches.
+//
+// Control flow follows EvaluatorJ3DK::numericalIntegration
+// (src/evaluators/evaluatorJ3DK.cu:895-1012): level 0 over all tasks, then
+// refinement iterations 1..maxLevel, each integrating the 4^level sub-panels
+// of the control panel of every task still unconverged and comparing with the
+// previous level by the Runge rule.  Level k is written to buf[k % 2], which
+// reproduces the reference's swapped result buffers exactly.  Unlike the
+// reference, child tasks are never materialised: a task at level k is one
+// group of lanes that walks its 4^k sub-panels by path digits and reduces in
+// a fixed order, so results are bitwise reproducible.
+#include "i2_internal.hpp"
+
+#include <algorithm>
+#include <fstream>
+
+namespace i2 {
+using namespace i2d;
+
+// ------------------------------------------------------------ task decoding
+__device__ __forceinline__ int find_row(const int64_t* __restrict__ row_start, int nT, int64_t k) {
+    int lo = 0, hi = nT - 1;   // largest i with row_start[i] <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_start[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// k-th disjoint (i<j) pair in lexicographic order
+__device__ __forceinline__ void decode_disjoint(const TaskSource& s, int64_t k, int* pi, int* pj) {
+    const int i = find_row(s.row_start, s.nT, k);
+    int64_t j = i + 1 + (k - s.row_start[i]);
+    for (int64_t q = s.nbr_start[i]; q < s.nbr_start[i + 1]; ++q) {
+        if (s.nbr[q] <= j) ++j; else break;
+    }
+    *pi = i; *pj = (int)j;
+}
+template <bool IMPLICIT>
+__device__ __forceinline__ void task_ij(const TaskSource& s, int64_t t, int* i, int* j) {
+    if (IMPLICIT) {
+        if (t < s.n_unordered) decode_disjoint(s, t, i, j);
+        else decode_disjoint(s, t - s.n_unordered, j, i);
+    } else {
+        *i = s.ij3[3 * t];
+        *j = s.ij3[3 * t + 1];
+    }
+}
+
+// ------------------------------------------------------------ per-pair integrand
+template <int TYPE> struct PairCtx;
+template <> struct PairCtx<I2_NOT_NEIGHBORS> {
+    __device__ __forceinline__ void init(const Panel&, const Panel&) {}
+    __device__ __forceinline__ q4 sing(d3) const { return q4zero(); }
+};
+template <> struct PairCtx<I2_ATTACHED_NEIGHBORS> {
+    EdgeCtx c;
+    __device__ __forceinline__ void init(const Panel& I, const Panel& J) {
+        int s1, s2;
+        shifts_edge(I, J, &s1, &s2);
+        c = edge_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3));
+    }
+    __device__ __forceinline__ q4 sing(d3 p) const { return sing_edge(p, c); }
+};
+template <> struct PairCtx<I2_SIMPLE_NEIGHBORS> {
+    VertexCtx c;
+    __device__ __forceinline__ void init(const Panel& I, const Panel& J) {
+        int s1, s2;
+        shifts_vertex(I, J, &s1, &s2);
+        c = vertex_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    }
+    __device__ __forceinline__ q4 sing(d3 p) const { return sing_vertex(p, c); }
+};
+
+// Quadrature over one sub-panel: area * sum_k w_k (thetaPsi - singular part)(r_k)
+// (NumericalIntegrator3d.cu:560-585, evaluatorJ3DK.cu:98-106)
+template <int TYPE>
+__device__ __forceinline__ q4 quad_sub(const Rule& R, d3 a, d3 b, d3 c, double area, const Panel& J, const PairCtx<TYPE>& ctx) {
+    q4 acc = q4zero();
+    for (int k = 0; k < R.n; ++k) {
+        d3 p = R.L1[k] * a;
+        p = p + R.L2[k] * b;
+        p = p + R.L3[k] * c;
+        q4 f = theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
+        if (TYPE != I2_NOT_NEIGHBORS) f = f - ctx.sing(p);
+        acc = acc + R.w[k] * f;
+    }
+    return area * acc;
+}
+
+// Level-`level` value of tasks: lanes per task = 1 << lanes_log2; lane `sub`
+// walks sub-panels sub, sub + G, ... < 4^level; butterfly reduction.
+template <int TYPE, bool IMPLICIT>
+__global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __restrict__ list, TaskSource src,
+                                               const Panel* __restrict__ panels, Rule R, int level, int lanes_log2,
+                                               double4* __restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int G = 1 << lanes_log2;
+    const int64_t slot = gid >> lanes_log2;
+    const int sub = (int)(gid & (G - 1));
+    const bool active = slot < nitems;
+    q4 acc = q4zero();
+    int64_t z = 0;
+    if (active) {
+        z = list ? (int64_t)list[slot] : slot;
+        int i, j;
+        task_ij<IMPLICIT>(src, z, &i, &j);
+        const Panel I = panels[i];
+        const Panel J = panels[j];
+        PairCtx<TYPE> ctx;
+        ctx.init(I, J);
+        double area = I.area;
+        for (int l = 0; l < level; ++l) area *= 0.25;
+        const uint32_t nsub = 1u << (2 * level);
+        for (uint32_t c = sub; c < nsub; c += G) {
+            d3 a = I.a, b = I.b, cc = I.c;
+            sub_panel(a, b, cc, level, c);
+            acc = acc + quad_sub<TYPE>(R, a, b, cc, area, J, ctx);
+        }
+    }
+    for (int m = 1; m < G; m <<= 1) {
+        acc.x += __shfl_xor(acc.x, m);
+        acc.y += __shfl_xor(acc.y, m);
+        acc.z += __shfl_xor(acc.z, m);
+        acc.w += __shfl_xor(acc.w, m);
+    }
+    if (active && sub == 0) out[z] = make_double4(acc.x, acc.y, acc.z, acc.w);
+}
+
+// Runge comparison of iteration `it` (evaluator3d.cu:76-99): items are all
+// tasks (list == nullptr) or the previous rest list.  Unconverged tasks are
+// appended to rest_out (one atomic per wave); with keep_converged the
+// converged value is also written to the other buffer so that it survives
+// later swaps (the reference's quirk disabled).
+__global__ __launch_bounds__(256) void k_compare(int64_t nitems, const int32_t* __restrict__ list, double4* __restrict__ cur,
+                                                 double4* __restrict__ prev, double pow2p, double tol, int keep_converged,
+                                                 int it, int8_t* __restrict__ conv, int32_t* __restrict__ rest_out,
+                                                 unsigned long long* __restrict__ counter) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = gid < nitems;
+    bool unconv = false;
+    int64_t z = 0;
+    if (active) {
+        z = list ? (int64_t)list[gid] : gid;
+        const double4 c4 = cur[z], p4 = prev[z];
+        const double crit = runge_criterion({c4.x, c4.y, c4.z, c4.w}, {p4.x, p4.y, p4.z, p4.w}, pow2p);
+        unconv = crit > tol;
+        conv[z] = (int8_t)it;
+        if (!unconv && keep_converged) prev[z] = c4;
+    }
+    const unsigned long long mask = __ballot(unconv);
+    const int lane = threadIdx.x & 63;
+    const int before = __popcll(mask & ((1ull << lane) - 1ull));
+    unsigned long long base = 0;
+    if (lane == 0 && mask) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+    base = __shfl(base, 0);
+    if (unconv) rest_out[base + before] = (int32_t)z;
+}
+
+// Analytic singular part (vertex/edge-sharing) and conversion to J
+// (evaluatorJ3DK.cu:22-56, 224-264).  The final integrals stay in `ints`.
+template <int TYPE, bool IMPLICIT>
+__global__ __launch_bounds__(256) void k_finalize(int64_t n, TaskSource src, const Panel* __restrict__ panels,
+                                                  double4* __restrict__ ints, double* __restrict__ results) {
+    const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= n) return;
+    int i, j;
+    task_ij<IMPLICIT>(src, z, &i, &j);
+    const Panel I = panels[i];
+    const Panel J = panels[j];
+    const double4 v4 = ints[z];
+    q4 v = {v4.x, v4.y, v4.z, v4.w};
+    d3 res;
+    if (TYPE == I2_ATTACHED_NEIGHBORS) {
+        int s1, s2;
+        shifts_edge(I, J, &s1, &s2);
+        v = v + integrate_sing_edge(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                    vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+        res = finalize_plain(v, J.n);
+    } else if (TYPE == I2_SIMPLE_NEIGHBORS) {
+        int s1, s2;
+        shifts_vertex(I, J, &s1, &s2);
+        v = v + integrate_sing_vertex(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                      vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+        res = finalize_vertex(v, J.n, I.area);
+    } else {
+        res = finalize_plain(v, J.n);
+    }
+    ints[z] = make_double4(v.x, v.y, v.z, v.w);
+    results[3 * z] = res.x; results[3 * z + 1] = res.y; results[3 * z + 2] = res.z;
+}
+
+// (i,j)/(j,i) symmetry error (evaluator3d.cu:45-57)
+__global__ void k_symmetry(int64_t n, const double* __restrict__ r, double* __restrict__ err) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double ax = r[3 * q], ay = r[3 * q + 1], az = r[3 * q + 2];
+    const double bx = r[3 * (n + q)], by = r[3 * (n + q) + 1], bz = r[3 * (n + q) + 2];
+    const double num = fabs(ax + bx) + fabs(ay + by) + fabs(az + bz);
+    const double den = fmax(fabs(ax) + fabs(ay) + fabs(az), fabs(bx) + fabs(by) + fabs(bz));
+    const double d = num / den;
+    err[q] = d;
+    err[n + q] = d;
+}
+
+// ------------------------------------------------------------ host orchestration
+static unsigned blocks_for(int64_t threads, int bs) { return (unsigned)((threads + bs - 1) / bs); }
+
+static int lanes_log2_for(int level) {
+    if (level <= 1) return 0;
+    return std::min(2 * level, 6);   // 16 lanes at level 2, a wavefront from level 3
+}
+
+template <int TYPE, bool IMPLICIT>
+static int launch_level(i2_evaluator* ev, int64_t nitems, const int32_t* list, const TaskSource& src, int level, double4* out) {
+    if (nitems == 0) return I2_OK;
+    const int ll = lanes_log2_for(level);
+    const int64_t threads = nitems << ll;
+    hipLaunchKernelGGL((k_level<TYPE, IMPLICIT>), dim3(blocks_for(threads, 256)), dim3(256), 0, ev->stream, nitems, list, src,
+                       ev->mesh->d_panels.p, ev->integ->rule, level, ll, out);
+    return check_launch("k_level");
+}
+
+template <int TYPE, bool IMPLICIT>
+static int run_type(i2_evaluator* ev, TypeState& ts, const TaskSource& src) {
+    const i2_integrator& it = *ev->integ;
+    const int64_t n = ts.n;
+    ts.K = 0; ts.final_buf = 0; ts.refined_tasks = 0; ts.ms = 0.0;
+    if (n == 0) return I2_OK;
+    if (n > (int64_t)INT32_MAX) { set_error("%lld tasks of one type exceed the plain path's 32-bit task lists", (long long)n); return I2_ERR_RANGE; }
+    I2_TRY(ts.buf[0].ensure(n));
+    I2_TRY(ts.buf[1].ensure(n));
+    I2_TRY(ts.results.ensure(3 * n));
+    I2_TRY(ts.conv.ensure(n));
+    I2_HIP(hipEventRecord(ev->ev0, ev->stream));
+    if (it.fixed) {
+        I2_HIP(hipMemsetAsync(ts.conv.p, it.fixed_level, (size_t)n, ev->stream));
+        I2_TRY((launch_level<TYPE, IMPLICIT>(ev, n, nullptr, src, it.fixed_level, ts.buf[0].p)));
+        ts.refined_tasks = n << (2 * it.fixed_level);
+        ts.K = it.fixed_level;
+        ts.final_buf = 0;
+    } else {
+        I2_TRY(ts.rest[0].ensure(n));
+        I2_TRY(ts.rest[1].ensure(n));
+        I2_HIP(hipMemsetAsync(ts.conv.p, 0, (size_t)n, ev->stream));
+        I2_TRY((launch_level<TYPE, IMPLICIT>(ev, n, nullptr, src, 0, ts.buf[0].p)));
+        ts.refined_tasks = n;
+        int64_t remaining = n;
+        int iter = 0, cur_rest = 0;
+        while (remaining > 0 && iter < it.max_level) {
+            ++iter;
+            const int32_t* list = (iter == 1) ? nullptr : ts.rest[cur_rest].p;
+            double4* cur = ts.buf[iter % 2].p;
+            double4* prev = ts.buf[(iter - 1) % 2].p;
+            I2_TRY((launch_level<TYPE, IMPLICIT>(ev, remaining, list, src, iter, cur)));
+            ts.refined_tasks += remaining << (2 * iter);
+            I2_HIP(hipMemsetAsync(ev->d_counter.p, 0, sizeof(unsigned long long), ev->stream));
+            hipLaunchKernelGGL(k_compare, dim3(blocks_for(remaining, 256)), dim3(256), 0, ev->stream, remaining, list, cur, prev,
+                               it.rule.pow2p, it.tol, it.quirk ? 0 : 1, iter, ts.conv.p, ts.rest[1 - cur_rest].p, ev->d_counter.p);
+            I2_TRY(check_launch("k_compare"));
+            I2_HIP(hipMemcpyAsync(ev->h_counter, ev->d_counter.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, ev->stream));
+            I2_HIP(hipStreamSynchronize(ev->stream));
+            remaining = (int64_t)*ev->h_counter;
+            cur_rest = 1 - cur_rest;
+        }
+        ts.K = iter;
+        ts.final_buf = iter % 2;
+    }
+    hipLaunchKernelGGL((k_finalize<TYPE, IMPLICIT>), dim3(blocks_for(n, 256)), dim3(256), 0, ev->stream, n, src,
+                       ev->mesh->d_panels.p, ts.buf[ts.final_buf].p, ts.results.p);
+    I2_TRY(check_launch("k_finalize"));
+    I2_HIP(hipEventRecord(ev->ev1, ev->stream));
+    I2_HIP(hipEventSynchronize(ev->ev1));
+    float ms = 0.f;
+    I2_HIP(hipEventElapsedTime(&ms, ev->ev0, ev->ev1));
+    ts.ms = ms;
+    return I2_OK;
+}
+
+int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check) {
+    const i2_mesh* m = ev->mesh;
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    ev->ran = false;
+    ev->all_pairs = all_pairs;
+    // types in the reference's order: vertex-sharing, edge-sharing, disjoint (evaluator3d.cu:173-186)
+    for (int t = 0; t < 3; ++t) {
+        TypeState& ts = ev->type[t];
+        TaskSource src;
+        src.nT = (int)m->nT;
+        ts.has_errors = false;
+        if (ts.implicit) {
+            src.n_unordered = m->pairs[I2_NOT_NEIGHBORS];
+            src.row_start = m->d_row_start.p;
+            src.nbr_start = m->d_nbr_start.p;
+            src.nbr = m->d_nbr.p;
+            I2_TRY((run_type<I2_NOT_NEIGHBORS, true>(ev, ts, src)));
+        } else {
+            src.ij3 = ts.tasks.p;
+            if (t == I2_SIMPLE_NEIGHBORS) I2_TRY((run_type<I2_SIMPLE_NEIGHBORS, false>(ev, ts, src)));
+            else if (t == I2_ATTACHED_NEIGHBORS) I2_TRY((run_type<I2_ATTACHED_NEIGHBORS, false>(ev, ts, src)));
+            else I2_TRY((run_type<I2_NOT_NEIGHBORS, false>(ev, ts, src)));
+        }
+        if (check && all_pairs && ts.n > 0) {
+            I2_TRY(ts.errors.ensure(ts.n));
+            const int64_t half = ts.n / 2;
+            hipLaunchKernelGGL(k_symmetry, dim3(blocks_for(half, 256)), dim3(256), 0, ev->stream, half, ts.results.p, ts.errors.p);
+            I2_TRY(check_launch("k_symmetry"));
+            ts.has_errors = true;
+        }
+    }
+    I2_HIP(hipStreamSynchronize(ev->stream));
+    ev->ran = true;
+    return I2_OK;
+}
+
+// Builds the ordered task list [(i,j,k)] ++ [(j,i,n+k)] of Evaluator3D::runAllPairs
+// (evaluator3d.cu:22-31, 156-169) on the host.
+static std::vector<int32_t> with_reversed(const std::vector<int32_t>& l) {
+    const size_t n = l.size() / 3;
+    std::vector<int32_t> out(6 * n);
+    for (size_t q = 0; q < n; ++q) {
+        out[3 * q] = l[3 * q]; out[3 * q + 1] = l[3 * q + 1]; out[3 * q + 2] = (int32_t)q;
+        out[3 * (n + q)] = l[3 * q + 1]; out[3 * (n + q) + 1] = l[3 * q]; out[3 * (n + q) + 2] = (int32_t)(n + q);
+    }
+    return out;
+}
+
+static int set_tasks(i2_evaluator* ev, int t, const std::vector<int32_t>& h) {
+    TypeState& ts = ev->type[t];
+    ts.implicit = false;
+    ts.n = (int64_t)h.size() / 3;
+    I2_TRY(ts.tasks.ensure(3 * ts.n));
+    if (ts.n) I2_HIP(hipMemcpy(ts.tasks.p, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
+    return I2_OK;
+}
+
+}  // namespace i2
+
+using namespace i2;
+
+extern "C" {
+
+int i2_integrator_create(i2_integrator** out, const i2_mesh* mesh, const double* L12, const double* w, int nPts, int order) {
+    if (!out || !mesh || !L12 || !w) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (nPts < 1 || nPts > MAX_POINTS) { set_error("rule must have 1..%d points", MAX_POINTS); return I2_ERR_INVALID; }
+    if (order < 0 || order > 30) { set_error("rule order out of range"); return I2_ERR_INVALID; }
+    i2_integrator* it = new i2_integrator;
+    it->mesh = mesh;
+    it->rule.n = nPts;
+    it->rule.order = order;
+    it->rule.pow2p = (double)(1 << order);
+    for (int k = 0; k < MAX_POINTS; ++k) {
+        if (k < nPts) {
+            it->rule.L1[k] = L12[2 * k];
+            it->rule.L2[k] = L12[2 * k + 1];
+            it->rule.L3[k] = 1.0 - L12[2 * k] - L12[2 * k + 1];
+            it->rule.w[k] = w[k];
+        } else {
+            it->rule.L1[k] = it->rule.L2[k] = it->rule.L3[k] = it->rule.w[k] = 0.0;
+        }
+    }
+    *out = it;
+    return I2_OK;
+}
+void i2_integrator_destroy(i2_integrator* it) { delete it; }
+
+int i2_integrator_set_fixed_refinement_level(i2_integrator* it, int level) {
+    if (!it) { set_error("null integrator"); return I2_ERR_INVALID; }
+    if (level < 0 || level > 12) { set_error("fixed refinement level %d out of range [0, 12]", level); return I2_ERR_RANGE; }
+    it->fixed = 1;
+    it->fixed_level = level;
+    return I2_OK;
+}
+int i2_integrator_set_error_control(i2_integrator* it, int maxLevel, double tol, int refParityQuirk) {
+    if (!it) { set_error("null integrator"); return I2_ERR_INVALID; }
+    if (maxLevel < 0 || maxLevel > 12) { set_error("maxLevel %d out of range [0, 12]", maxLevel); return I2_ERR_RANGE; }
+    if (!(tol > 0)) { set_error("tolerance must be positive"); return I2_ERR_INVALID; }
+    it->fixed = 0;
+    it->max_level = maxLevel;
+    it->tol = tol;
+    it->quirk = refParityQuirk ? 1 : 0;
+    return I2_OK;
+}
+
+int i2_evaluator_create(i2_evaluator** out, const i2_mesh* m, i2_integrator* it) {
+    if (!out || !m || !it) { set_error("null argument"); return I2_ERR_INVALID; }
+    *out = nullptr;
+    if (!m->prepared) { set_error("mesh not prepared (call i2_mesh_prepare)"); return I2_ERR_STATE; }
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    i2_evaluator* ev = new i2_evaluator;
+    ev->mesh = m;
+    ev->integ = it;
+    if (hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ev->ev0) != hipSuccess || hipEventCreate(&ev->ev1) != hipSuccess ||
+        hipHostMalloc((void**)&ev->h_counter, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("failed to create evaluator stream/events");
+        i2_evaluator_destroy(ev);
+        return I2_ERR_HIP;
+    }
+    if (ev->d_counter.ensure(1) != I2_OK) { i2_evaluator_destroy(ev); return I2_ERR_ALLOC; }
+    *out = ev;
+    return I2_OK;
+}
+
+void i2_evaluator_destroy(i2_evaluator* ev) {
+    if (!ev) return;
+    {
+        DeviceGuard g(ev->mesh->device);
+        if (ev->stream) (void)hipStreamSynchronize(ev->stream);
+        for (auto& ts : ev->type) {
+            ts.tasks.reset(); ts.buf[0].reset(); ts.buf[1].reset(); ts.results.reset(); ts.errors.reset();
+            ts.conv.reset(); ts.rest[0].reset(); ts.rest[1].reset();
+        }
+        ev->d_counter.reset();
+        if (ev->h_counter) (void)hipHostFree(ev->h_counter);
+        if (ev->ev0) (void)hipEventDestroy(ev->ev0);
+        if (ev->ev1) (void)hipEventDestroy(ev->ev1);
+        if (ev->stream) (void)hipStreamDestroy(ev->stream);
+    }
+    delete ev;
+}
+
+int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
+    if (!ev) { set_error("null evaluator"); return I2_ERR_INVALID; }
+    const i2_mesh* m = ev->mesh;
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_TRY(set_tasks(ev, I2_SIMPLE_NEIGHBORS, with_reversed(m->h_simple)));
+    I2_TRY(set_tasks(ev, I2_ATTACHED_NEIGHBORS, with_reversed(m->h_attached)));
+    TypeState& d = ev->type[I2_NOT_NEIGHBORS];
+    d.implicit = true;
+    d.n = 2 * m->pairs[I2_NOT_NEIGHBORS];
+    d.tasks.reset();
+    return evaluator_run(ev, true, checkCorrectness != 0);
+}
+
+int i2_evaluator_run_pairs(i2_evaluator* ev, const int32_t* simple3, int64_t nSimple, const int32_t* attached3, int64_t nAttached,
+                           const int32_t* not3, int64_t nNot) {
+    if (!ev) { set_error("null evaluator"); return I2_ERR_INVALID; }
+    if ((nSimple && !simple3) || (nAttached && !attached3) || (nNot && !not3) || nSimple < 0 || nAttached < 0 || nNot < 0) {
+        set_error("invalid task lists");
+        return I2_ERR_INVALID;
+    }
+    const i2_mesh* m = ev->mesh;
+    const int32_t* lists[3] = {simple3, attached3, not3};
+    const int64_t ns[3] = {nSimple, nAttached, nNot};
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    for (int t = 0; t < 3; ++t) {
+        std::vector<int32_t> h(lists[t] ? lists[t] : (const int32_t*)nullptr, lists[t] ? lists[t] + 3 * ns[t] : (const int32_t*)nullptr);
+        for (int64_t q = 0; q < ns[t]; ++q) {
+            const int32_t i = h[3 * q], j = h[3 * q + 1];
+            if (i < 0 || j < 0 || i >= m->nT || j >= m->nT) { set_error("task %lld of type %d references a cell outside [0, %lld)", (long long)q, t, (long long)m->nT); return I2_ERR_INVALID; }
+            // the integrand of each type is only valid for pairs of that type
+            const int32_t* a = &m->h_tri[3 * (int64_t)i];
+            const int32_t* b = &m->h_tri[3 * (int64_t)j];
+            int cnt = 0;
+            for (int x = 0; x < 3; ++x) if (a[x] == b[0] || a[x] == b[1] || a[x] == b[2]) ++cnt;
+            const int expect = (t == I2_SIMPLE_NEIGHBORS) ? 1 : (t == I2_ATTACHED_NEIGHBORS ? 2 : 0);
+            if (cnt != expect) { set_error("task %lld (%d, %d) shares %d vertices but was given as type %d", (long long)q, i, j, cnt, t); return I2_ERR_TOPOLOGY; }
+            h[3 * q + 2] = (int32_t)q;
+        }
+        I2_TRY(set_tasks(ev, t, h));
+    }
+    return evaluator_run(ev, false, false);
+}
+
+int i2_evaluator_task_count(const i2_evaluator* ev, int type, int64_t* n) {
+    if (!ev || !n || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    *n = ev->type[type].n;
+    return I2_OK;
+}
+
+int i2_evaluator_get_results(const i2_evaluator* ev, int type, int32_t* tasks3, double* integrals4, double* results3,
+                             double* errors, int8_t* convLevel) {
+    if (!ev || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!ev->ran) { set_error("no results: run the evaluator first"); return I2_ERR_STATE; }
+    const TypeState& ts = ev->type[type];
+    const i2_mesh* m = ev->mesh;
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    const int64_t n = ts.n;
+    if (n == 0) return I2_OK;
+    if (tasks3) {
+        if (ts.implicit) {
+            const int64_t h = n / 2;
+            std::vector<int32_t> l(3 * h);
+            I2_TRY(i2_mesh_copy_neighbors(m, I2_NOT_NEIGHBORS, l.data()));
+            for (int64_t q = 0; q < h; ++q) {
+                tasks3[3 * q] = l[3 * q]; tasks3[3 * q + 1] = l[3 * q + 1]; tasks3[3 * q + 2] = (int32_t)q;
+                tasks3[3 * (h + q)] = l[3 * q + 1]; tasks3[3 * (h + q) + 1] = l[3 * q]; tasks3[3 * (h + q) + 2] = (int32_t)(h + q);
+            }
+        } else {
+            I2_HIP(hipMemcpy(tasks3, ts.tasks.p, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost));
+        }
+    }
+    if (integrals4) I2_HIP(hipMemcpy(integrals4, ts.buf[ts.final_buf].p, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    if (results3) I2_HIP(hipMemcpy(results3, ts.results.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (errors) {
+        if (!ts.has_errors) { set_error("symmetry errors were not computed (runAllPairs with checkCorrectness)"); return I2_ERR_STATE; }
+        I2_HIP(hipMemcpy(errors, ts.errors.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    if (convLevel) I2_HIP(hipMemcpy(convLevel, ts.conv.p, (size_t)n, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+int i2_evaluator_refinement_iterations(const i2_evaluator* ev, int type, int* K) {
+    if (!ev || !K || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!ev->ran) { set_error("no results: run the evaluator first"); return I2_ERR_STATE; }
+    *K = ev->type[type].K;
+    return I2_OK;
+}
+
+int i2_evaluator_type_stats(const i2_evaluator* ev, int type, double* ms, int64_t* refinedTasks) {
+    if (!ev || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!ev->ran) { set_error("no results: run the evaluator first"); return I2_ERR_STATE; }
+    if (ms) *ms = ev->type[type].ms;
+    if (refinedTasks) *refinedTasks = ev->type[type].refined_tasks;
+    return I2_OK;
+}
+
+int i2_evaluator_output_results_to_file(const i2_evaluator* ev, int type, int format, const char* dir) {
+    if (!ev || type < 0 || type > 2 || (format != I2_OUTPUT_PLAIN_TEXT && format != I2_OUTPUT_CSV)) {
+        set_error("invalid argument");
+        return I2_ERR_INVALID;
+    }
+    if (!ev->ran) { set_error("no results: run the evaluator first"); return I2_ERR_STATE; }
+    const TypeState& ts = ev->type[type];
+    if (ts.n == 0) { set_error("no tasks of type %d", type); return I2_ERR_STATE; }   // the reference returns false
+    std::vector<int32_t> tasks(3 * ts.n);
+    std::vector<double> res(3 * ts.n), err;
+    I2_TRY(i2_evaluator_get_results(ev, type, tasks.data(), nullptr, res.data(), nullptr, nullptr));
+    if (ts.has_errors) { err.resize(ts.n); I2_TRY(i2_evaluator_get_results(ev, type, nullptr, nullptr, nullptr, err.data(), nullptr)); }
+    static const char* names[3] = {"SimpleNeighbors", "AttachedNeighbors", "NotNeighbors"};
+    std::string path = dir ? std::string(dir) + "/" : std::string();
+    path += names[type];
+    path += (format == I2_OUTPUT_CSV) ? ".csv" : ".dat";
+    std::ofstream f(path);
+    if (!f.is_open()) { set_error("cannot open %s for writing", path.c_str()); return I2_ERR_IO; }
+    // layout of evaluator3d.cu:418-441 (default stream precision)
+    if (format == I2_OUTPUT_CSV) {
+        f << "\"TaskI\";\"TaskJ\";\"IntegralX\";\"IntegralY\";\"IntegralZ\"";
+        if (ts.has_errors) f << ";\"Error\"";
+        f << "\n";
+    }
+    for (int64_t q = 0; q < ts.n; ++q) {
+        if (format == I2_OUTPUT_PLAIN_TEXT) {
+            f << "(" << tasks[3 * q] << ", " << tasks[3 * q + 1] << "): [" << res[3 * q] << ", " << res[3 * q + 1] << ", " << res[3 * q + 2] << "]";
+            if (ts.has_errors) f << ", error = " << err[q];
+        } else {
+            f << tasks[3 * q] << ";" << tasks[3 * q + 1] << ";" << res[3 * q] << ";" << res[3 * q + 1] << ";" << res[3 * q + 2];
+            if (ts.has_errors) f << ";" << err[q];
+        }
+        f << "\n";
+    }
+    if (!f.good()) { set_error("write to %s failed", path.c_str()); return I2_ERR_IO; }
+    return I2_OK;
+}
+
+}  // extern "C"

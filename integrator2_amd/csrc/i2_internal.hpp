@@ -1,0 +1,164 @@
+// i2_internal.hpp -- host-side objects behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/integrator2_mi355x.h"
+#include "i2_device.hpp"
+
+namespace i2 {
+
+// ------------------------------------------------------------ errors
+void set_error(const char* fmt, ...);
+void clear_error();
+
+// Device buffer with RAII ownership; never copied (the reference's
+// deviceVector is copyable and would double-free, src/common/device_vector.cuh:15-109).
+template <class T> struct DBuf {
+    T* p = nullptr;
+    int64_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept { if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; } return *this; }
+    ~DBuf() { reset(); }
+    void reset() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    // (re)allocates when the capacity is too small; contents are not kept
+    int ensure(int64_t count) {
+        if (count <= n && p) return I2_OK;
+        reset();
+        if (count == 0) return I2_OK;
+        if (hipMalloc((void**)&p, sizeof(T) * (size_t)count) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            set_error("hipMalloc of %lld bytes failed", (long long)(sizeof(T) * (size_t)count));
+            return I2_ERR_ALLOC;
+        }
+        n = count;
+        return I2_OK;
+    }
+};
+
+#define I2_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            i2::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return I2_ERR_HIP;                                                                 \
+        }                                                                                      \
+    } while (0)
+#define I2_TRY(expr)            \
+    do {                        \
+        int s_ = (expr);        \
+        if (s_ != I2_OK) return s_; \
+    } while (0)
+
+// Makes `device` current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) { (void)hipGetLastError(); prev = -1; }
+        ok = hipSetDevice(dev) == hipSuccess;
+        if (!ok) { (void)hipGetLastError(); set_error("hipSetDevice(%d) failed", dev); }
+    }
+    ~DeviceGuard() { if (prev >= 0) (void)hipSetDevice(prev); }
+};
+
+inline int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_error("launch of %s failed: %s", what, hipGetErrorString(e)); return I2_ERR_HIP; }
+    return I2_OK;
+}
+
+// ------------------------------------------------------------ mesh
+}  // namespace i2
+
+struct i2_mesh {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool prepared = false;
+    int64_t nV = 0, nT = 0;
+    std::vector<double> h_xyz;          // host copy of vertices (AoS)
+    std::vector<int32_t> h_tri;         // host copy of cells (AoS, 0-based)
+    // neighbour lists (i<j, k) for vertex- and edge-sharing pairs, lexicographic
+    std::vector<int32_t> h_simple, h_attached;
+    int64_t pairs[3] = {0, 0, 0};
+    // per-cell sorted neighbour indices j > i (CSR) and disjoint-pair row offsets:
+    // the k-th disjoint (i<j) pair in lexicographic order is found from these
+    std::vector<int64_t> h_nbr_start, h_row_start;
+    std::vector<int32_t> h_nbr;
+    // device arrays
+    i2::DBuf<double> d_xyz;             // vertices (double3 AoS)
+    i2::DBuf<int32_t> d_tri;            // cells (int3 AoS)
+    i2::DBuf<double> d_normals;         // double3 AoS
+    i2::DBuf<double> d_measures;
+    i2::DBuf<i2d::Panel> d_panels;      // kernel-side panel records
+    i2::DBuf<int32_t> d_simple, d_attached;
+    i2::DBuf<int64_t> d_nbr_start, d_row_start;
+    i2::DBuf<int32_t> d_nbr;
+};
+
+struct i2_integrator {
+    const i2_mesh* mesh = nullptr;
+    i2d::Rule rule{};
+    int fixed = 0;          // 1: fixed refinement level
+    int fixed_level = 0;
+    int max_level = 5;      // MAX_REFINE_LEVEL (src/common/constants.h:56)
+    double tol = 1e-5;      // EPS_INTEGRATION (src/common/constants.h:18)
+    int quirk = 1;          // reproduce the result-buffer swap
+};
+
+namespace i2 {
+
+// Task source of one neighbour type: explicit ordered (i, j) list or, for
+// all disjoint pairs, implicit decoding of task index t into (i, j).
+struct TaskSource {
+    const int32_t* ij3 = nullptr;     // explicit: (i, j, k) per task
+    int64_t n_unordered = 0;          // implicit: number of i<j pairs; t >= n -> reversed
+    const int64_t* row_start = nullptr;
+    const int64_t* nbr_start = nullptr;
+    const int32_t* nbr = nullptr;
+    int nT = 0;
+};
+
+struct TypeState {
+    int64_t n = 0;                 // ordered tasks
+    bool implicit = false;         // disjoint all-pairs: tasks decoded on the fly
+    DBuf<int32_t> tasks;           // explicit (i, j, k)
+    DBuf<double4> buf[2];          // level values, level k written to buf[k % 2]
+    DBuf<double> results;          // J (double3 AoS)
+    DBuf<double> errors;           // symmetry error
+    DBuf<int8_t> conv;             // convergence iteration
+    DBuf<int32_t> rest[2];         // rest lists
+    int K = 0;                     // refinement iterations run
+    int final_buf = 0;             // buffer holding the final integrals
+    double ms = 0.0;
+    int64_t refined_tasks = 0;
+    bool has_errors = false;
+};
+
+}  // namespace i2
+
+struct i2_evaluator {
+    const i2_mesh* mesh = nullptr;
+    i2_integrator* integ = nullptr;
+    hipStream_t stream = nullptr;
+    bool ran = false;
+    bool all_pairs = false;
+    i2::TypeState type[3];
+    i2::DBuf<unsigned long long> d_counter;
+    unsigned long long* h_counter = nullptr;   // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace i2 {
+int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check);
+int mesh_upload_and_prepare(i2_mesh* m);
+}

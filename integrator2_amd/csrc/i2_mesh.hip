@@ -1,0 +1,304 @@
+// i2_mesh.hip -- Mesh3D behind the C ABI: file loading, upload, the panel
+// table kernel (normals, areas, edge tangents) and the neighbour lists.
+#include "i2_internal.hpp"
+
+#include <algorithm>
+#include <fstream>
+#include <sstream>
+
+namespace i2 {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+void clear_error() { g_last_error.clear(); }
+
+// One thread per panel: vertices, tangents of the inner integral, unit
+// normal n = (v2-v1)x(v3-v1)/|.| and area |.|/2 (src/Mesh3d.cu:23-32, 62-71).
+__global__ void k_panel_table(int64_t nT, const double* __restrict__ xyz, const int32_t* __restrict__ tri,
+                              i2d::Panel* __restrict__ panels, double* __restrict__ normals, double* __restrict__ measures) {
+    using namespace i2d;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nT) return;
+    const int v0 = tri[3 * c], v1 = tri[3 * c + 1], v2 = tri[3 * c + 2];
+    const d3 A = {xyz[3 * v0], xyz[3 * v0 + 1], xyz[3 * v0 + 2]};
+    const d3 B = {xyz[3 * v1], xyz[3 * v1 + 1], xyz[3 * v1 + 2]};
+    const d3 C = {xyz[3 * v2], xyz[3 * v2 + 1], xyz[3 * v2 + 2]};
+    Panel P;
+    P.a = A; P.b = B; P.c = C;
+    P.ta = unit(C - B);
+    P.tb = unit(A - C);
+    P.tc = unit(B - A);
+    const d3 cr = cross(B - A, C - A);
+    P.n = unit(cr);
+    P.area = vlen(cr) * 0.5;
+    P.v0 = v0; P.v1 = v1; P.v2 = v2; P.pad = 0;
+    panels[c] = P;
+    normals[3 * c] = P.n.x; normals[3 * c + 1] = P.n.y; normals[3 * c + 2] = P.n.z;
+    measures[c] = P.area;
+}
+
+// Neighbour classification by shared vertex indices (src/Mesh3d.cu:93-142)
+// through vertex->panel incidence, O(T * valence) instead of O(T^2): every
+// pair (i<j) sharing no vertex is disjoint.  Lists are lexicographic.
+static int classify(i2_mesh* m) {
+    const int64_t nT = m->nT, nV = m->nV;
+    std::vector<int64_t> inc_start(nV + 1, 0);
+    for (int64_t c = 0; c < nT; ++c)
+        for (int k = 0; k < 3; ++k) inc_start[m->h_tri[3 * c + k] + 1]++;
+    for (int64_t v = 0; v < nV; ++v) inc_start[v + 1] += inc_start[v];
+    std::vector<int32_t> inc(inc_start[nV]);
+    std::vector<int64_t> fill(inc_start.begin(), inc_start.end() - 1);
+    for (int64_t c = 0; c < nT; ++c)
+        for (int k = 0; k < 3; ++k) inc[fill[m->h_tri[3 * c + k]]++] = (int32_t)c;
+    m->h_simple.clear(); m->h_attached.clear();
+    m->h_nbr_start.assign(nT + 1, 0);
+    m->h_nbr.clear();
+    m->h_row_start.assign(nT + 1, 0);
+    std::vector<int32_t> cand;
+    for (int64_t i = 0; i < nT; ++i) {
+        cand.clear();
+        for (int k = 0; k < 3; ++k) {
+            const int32_t v = m->h_tri[3 * i + k];
+            for (int64_t q = inc_start[v]; q < inc_start[v + 1]; ++q)
+                if (inc[q] > i) cand.push_back(inc[q]);
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        int64_t excluded = 0;
+        for (int32_t j : cand) {
+            int cnt = 0;
+            for (int a = 0; a < 3; ++a) {
+                const int32_t va = m->h_tri[3 * i + a];
+                if (va == m->h_tri[3 * j] || va == m->h_tri[3 * j + 1] || va == m->h_tri[3 * j + 2]) ++cnt;
+            }
+            if (cnt == 1) { m->h_simple.push_back((int32_t)i); m->h_simple.push_back(j); m->h_simple.push_back(0); }
+            else if (cnt == 2) { m->h_attached.push_back((int32_t)i); m->h_attached.push_back(j); m->h_attached.push_back(0); }
+            // every candidate shares a vertex, so it is excluded from the disjoint row
+            m->h_nbr.push_back(j);
+            ++excluded;
+        }
+        m->h_nbr_start[i + 1] = (int64_t)m->h_nbr.size();
+        m->h_row_start[i + 1] = m->h_row_start[i] + (nT - 1 - i) - excluded;
+    }
+    for (size_t q = 0; q < m->h_simple.size() / 3; ++q) m->h_simple[3 * q + 2] = (int32_t)q;
+    for (size_t q = 0; q < m->h_attached.size() / 3; ++q) m->h_attached[3 * q + 2] = (int32_t)q;
+    m->pairs[I2_SIMPLE_NEIGHBORS] = (int64_t)m->h_simple.size() / 3;
+    m->pairs[I2_ATTACHED_NEIGHBORS] = (int64_t)m->h_attached.size() / 3;
+    m->pairs[I2_NOT_NEIGHBORS] = m->h_row_start[nT];
+    return I2_OK;
+}
+
+template <class T>
+static int upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t s) {
+    I2_TRY(d.ensure((int64_t)h.size()));
+    if (!h.empty()) I2_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return I2_OK;
+}
+
+int mesh_upload_and_prepare(i2_mesh* m) {
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    if (m->nT <= 0 || m->nV <= 0) { set_error("mesh is empty"); return I2_ERR_STATE; }
+    if (m->nT > (int64_t)INT32_MAX) { set_error("more than 2^31-1 cells"); return I2_ERR_RANGE; }
+    I2_TRY(classify(m));
+    I2_TRY(upload(m->d_xyz, m->h_xyz, m->stream));
+    I2_TRY(upload(m->d_tri, m->h_tri, m->stream));
+    I2_TRY(m->d_normals.ensure(3 * m->nT));
+    I2_TRY(m->d_measures.ensure(m->nT));
+    I2_TRY(m->d_panels.ensure(m->nT));
+    const int threads = 256;
+    const int64_t blocks = (m->nT + threads - 1) / threads;
+    hipLaunchKernelGGL(k_panel_table, dim3((unsigned)blocks), dim3(threads), 0, m->stream, m->nT, m->d_xyz.p, m->d_tri.p,
+                       m->d_panels.p, m->d_normals.p, m->d_measures.p);
+    I2_TRY(check_launch("k_panel_table"));
+    I2_TRY(upload(m->d_simple, m->h_simple, m->stream));
+    I2_TRY(upload(m->d_attached, m->h_attached, m->stream));
+    I2_TRY(upload(m->d_nbr_start, m->h_nbr_start, m->stream));
+    I2_TRY(upload(m->d_row_start, m->h_row_start, m->stream));
+    I2_TRY(upload(m->d_nbr, m->h_nbr, m->stream));
+    I2_HIP(hipStreamSynchronize(m->stream));
+    m->prepared = true;
+    return I2_OK;
+}
+
+}  // namespace i2
+
+using namespace i2;
+
+extern "C" {
+
+const char* i2_last_error(void) { return g_last_error.c_str(); }
+
+int i2_device_count(int* n) {
+    if (!n) { set_error("null pointer"); return I2_ERR_INVALID; }
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) { (void)hipGetLastError(); c = 0; }
+    *n = c;
+    return I2_OK;
+}
+
+int i2_mesh_create(i2_mesh** out, int device) {
+    if (!out) { set_error("null pointer"); return I2_ERR_INVALID; }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device available");
+        return I2_ERR_HIP;
+    }
+    if (device < 0 || device >= count) { set_error("device %d out of range (%d devices)", device, count); return I2_ERR_INVALID; }
+    DeviceGuard g(device);
+    if (!g.ok) return I2_ERR_HIP;
+    i2_mesh* m = new i2_mesh;
+    m->device = device;
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        delete m;
+        set_error("hipStreamCreate failed");
+        return I2_ERR_HIP;
+    }
+    *out = m;
+    return I2_OK;
+}
+
+void i2_mesh_destroy(i2_mesh* m) {
+    if (!m) return;
+    {
+        DeviceGuard g(m->device);
+        (void)hipStreamSynchronize(m->stream);
+        m->d_xyz.reset(); m->d_tri.reset(); m->d_normals.reset(); m->d_measures.reset(); m->d_panels.reset();
+        m->d_simple.reset(); m->d_attached.reset(); m->d_nbr_start.reset(); m->d_row_start.reset(); m->d_nbr.reset();
+        (void)hipStreamDestroy(m->stream);
+    }
+    delete m;
+}
+
+int i2_mesh_set_data(i2_mesh* m, const double* xyz, int64_t nV, const int32_t* tri, int64_t nT) {
+    if (!m || !xyz || !tri || nV <= 0 || nT <= 0) { set_error("invalid mesh data"); return I2_ERR_INVALID; }
+    if (nV > (int64_t)INT32_MAX || nT > (int64_t)INT32_MAX) { set_error("mesh too large for 32-bit indices"); return I2_ERR_RANGE; }
+    for (int64_t q = 0; q < 3 * nT; ++q)
+        if (tri[q] < 0 || tri[q] >= nV) { set_error("triangle %lld references vertex %d outside [0, %lld)", (long long)(q / 3), tri[q], (long long)nV); return I2_ERR_INVALID; }
+    m->nV = nV; m->nT = nT;
+    m->h_xyz.assign(xyz, xyz + 3 * nV);
+    m->h_tri.assign(tri, tri + 3 * nT);
+    m->prepared = false;
+    return I2_OK;
+}
+
+int i2_mesh_load_file(i2_mesh* m, const char* path, double scale) {
+    if (!m || !path) { set_error("null argument"); return I2_ERR_INVALID; }
+    std::ifstream f(path);
+    if (!f.is_open()) { set_error("cannot open mesh file %s", path); return I2_ERR_IO; }
+    long long nv = 0, nc = 0;
+    if (!(f >> nv >> nc) || nv <= 0 || nc < 0) { set_error("bad header in %s", path); return I2_ERR_IO; }
+    std::vector<double> xyz(3 * nv);
+    for (long long q = 0; q < nv; ++q) {
+        long long idx;
+        double x, y, z;
+        if (!(f >> idx >> x >> y >> z)) { set_error("truncated vertex list in %s", path); return I2_ERR_IO; }
+        xyz[3 * q] = scale * x; xyz[3 * q + 1] = scale * y; xyz[3 * q + 2] = scale * z;
+    }
+    // element records "idx type ..." (src/Mesh3d.cu:181-197): type 203 is a
+    // triangle with three 1-based vertex indices; other entities carry two
+    // more numbers and reduce the declared cell count.
+    std::vector<int32_t> tri;
+    long long declared = nc;
+    long long idx, type;
+    while (f >> idx >> type) {
+        if (type == 203) {
+            long long a, b, c;
+            if (!(f >> a >> b >> c)) break;
+            tri.push_back((int32_t)(a - 1)); tri.push_back((int32_t)(b - 1)); tri.push_back((int32_t)(c - 1));
+        } else {
+            --declared;
+            long long s1, s2;
+            if (!(f >> s1 >> s2)) break;
+        }
+    }
+    const long long ncells = std::min<long long>(declared, (long long)tri.size() / 3);
+    if (ncells <= 0) { set_error("no triangles in %s", path); return I2_ERR_IO; }
+    tri.resize(3 * ncells);
+    return i2_mesh_set_data(m, xyz.data(), nv, tri.data(), ncells);
+}
+
+int i2_mesh_prepare(i2_mesh* m) {
+    if (!m) { set_error("null mesh"); return I2_ERR_INVALID; }
+    return mesh_upload_and_prepare(m);
+}
+
+int i2_mesh_sizes(const i2_mesh* m, int64_t* nV, int64_t* nT, int64_t pairs[3]) {
+    if (!m) { set_error("null mesh"); return I2_ERR_INVALID; }
+    if (nV) *nV = m->nV;
+    if (nT) *nT = m->nT;
+    if (pairs) for (int t = 0; t < 3; ++t) pairs[t] = m->pairs[t];
+    return I2_OK;
+}
+
+int i2_mesh_copy_vertices(const i2_mesh* m, double* xyz) {
+    if (!m || !xyz) { set_error("null argument"); return I2_ERR_INVALID; }
+    std::copy(m->h_xyz.begin(), m->h_xyz.end(), xyz);
+    return I2_OK;
+}
+int i2_mesh_copy_cells(const i2_mesh* m, int32_t* tri) {
+    if (!m || !tri) { set_error("null argument"); return I2_ERR_INVALID; }
+    std::copy(m->h_tri.begin(), m->h_tri.end(), tri);
+    return I2_OK;
+}
+static int copy_d2h(const void* d, void* h, size_t bytes, int device) {
+    DeviceGuard g(device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_HIP(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+int i2_mesh_copy_normals(const i2_mesh* m, double* n3) {
+    if (!m || !n3) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!m->prepared) { set_error("mesh not prepared"); return I2_ERR_STATE; }
+    return copy_d2h(m->d_normals.p, n3, sizeof(double) * 3 * m->nT, m->device);
+}
+int i2_mesh_copy_measures(const i2_mesh* m, double* s) {
+    if (!m || !s) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!m->prepared) { set_error("mesh not prepared"); return I2_ERR_STATE; }
+    return copy_d2h(m->d_measures.p, s, sizeof(double) * m->nT, m->device);
+}
+int i2_mesh_copy_neighbors(const i2_mesh* m, int type, int32_t* ijk) {
+    if (!m || !ijk) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!m->prepared) { set_error("mesh not prepared"); return I2_ERR_STATE; }
+    if (type == I2_SIMPLE_NEIGHBORS) { std::copy(m->h_simple.begin(), m->h_simple.end(), ijk); return I2_OK; }
+    if (type == I2_ATTACHED_NEIGHBORS) { std::copy(m->h_attached.begin(), m->h_attached.end(), ijk); return I2_OK; }
+    if (type != I2_NOT_NEIGHBORS) { set_error("bad neighbour type %d", type); return I2_ERR_INVALID; }
+    int64_t k = 0;
+    for (int64_t i = 0; i < m->nT; ++i) {
+        int64_t q = m->h_nbr_start[i];
+        const int64_t qe = m->h_nbr_start[i + 1];
+        for (int64_t j = i + 1; j < m->nT; ++j) {
+            if (q < qe && m->h_nbr[q] == j) { ++q; continue; }
+            ijk[3 * k] = (int32_t)i; ijk[3 * k + 1] = (int32_t)j; ijk[3 * k + 2] = (int32_t)k;
+            ++k;
+        }
+    }
+    return I2_OK;
+}
+int i2_mesh_device_array(const i2_mesh* m, int what, const void** dptr, int64_t* n) {
+    if (!m || !dptr || !n) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!m->prepared) { set_error("mesh not prepared"); return I2_ERR_STATE; }
+    switch (what) {
+        case 0: *dptr = m->d_xyz.p; *n = m->nV; break;
+        case 1: *dptr = m->d_tri.p; *n = m->nT; break;
+        case 2: *dptr = m->d_normals.p; *n = m->nT; break;
+        case 3: *dptr = m->d_measures.p; *n = m->nT; break;
+        case 4: *dptr = m->d_simple.p; *n = m->pairs[I2_SIMPLE_NEIGHBORS]; break;
+        case 5: *dptr = m->d_attached.p; *n = m->pairs[I2_ATTACHED_NEIGHBORS]; break;
+        default: set_error("bad array id %d", what); return I2_ERR_INVALID;
+    }
+    return I2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,565 @@
+// i2_oracle.hpp -- CPU restatement of Integrator 2.0's panel-pair integration.
+//
+// TEST INFRASTRUCTURE ONLY.  This header is the checker for the HIP product
+// path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// may load the library built from it.  Nothing in integrator2_amd/ includes
+// or links it.
+//
+// Every routine is templated on the scalar type T so that the same source runs
+// with T = double (the FP64 oracle) and with T = Dual (forward-mode derivative
+// oracle for the adjoint).  Each function cites the reference file:line whose
+// formula it restates; operation order follows the reference expression so
+// that rounding stays close to it.
+//
+// Parity status of each part (how it is pinned) is listed in DESIGN.md of the
+// repository; functions whose only pin is the brute-force cubature carry the
+// note "pinned by cubature" below.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <vector>
+#include <algorithm>
+
+namespace i2o {
+
+// ---------------------------------------------------------------- constants
+// src/common/constants.h:13-24
+constexpr double DOUBLE_MIN = 2e-6;
+constexpr double EPS_ZERO = 1e-6;
+constexpr double EPS_ZERO2 = 1e-10;
+constexpr double EPS_PSI_THETA = EPS_ZERO;
+constexpr double EPS_PSI_THETA2 = EPS_PSI_THETA * EPS_PSI_THETA;
+constexpr double PI = 3.14159265358979323846;
+constexpr double TWO_PI = 6.28318530717958647692;
+constexpr double RECIPROCAL_FOUR_PI = 0.079577471545947667884;
+
+// ---------------------------------------------------------------- dual numbers
+// Forward-mode dual number: value v and one directional derivative d.
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double x) : v(x), d(0.0) {}
+    Dual(double x, double dx) : v(x), d(dx) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) { double q = a.v / b.v; return {q, (a.d - q * b.d) / b.v}; }
+inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
+inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+inline Dual& operator/=(Dual& a, Dual b) { a = a / b; return a; }
+inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
+inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
+inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
+inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
+inline Dual sqrt(Dual a) { double s = std::sqrt(a.v); return {s, a.d / (2.0 * s)}; }
+inline Dual log(Dual a) { return {std::log(a.v), a.d / a.v}; }
+inline Dual sin(Dual a) { return {std::sin(a.v), a.d * std::cos(a.v)}; }
+inline Dual cos(Dual a) { return {std::cos(a.v), -a.d * std::sin(a.v)}; }
+inline Dual tan(Dual a) { double t = std::tan(a.v); return {t, a.d * (1.0 + t * t)}; }
+inline Dual acos(Dual a) { return {std::acos(a.v), -a.d / std::sqrt(1.0 - a.v * a.v)}; }
+inline Dual atan2(Dual y, Dual x) {
+    double r2 = x.v * x.v + y.v * y.v;
+    return {std::atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / r2};
+}
+inline Dual fabs(Dual a) { return a.v < 0 ? -a : a; }
+
+inline double val(double x) { return x; }
+inline long double val(long double x) { return x; }
+inline double val(const Dual& x) { return x.v; }
+using std::sqrt; using std::log; using std::sin; using std::cos; using std::tan;
+using std::acos; using std::atan2; using std::fabs;
+
+template <class T> inline void sincos_t(T x, T* s, T* c) { *s = sin(x); *c = cos(x); }
+
+// ---------------------------------------------------------------- vectors
+// Point3 = double3 and the double4 (Psi, Theta) pair: src/common/cuda_math.cuh:13-300
+template <class T> struct V3 { T x, y, z; };
+template <class T> struct Q4 { T x, y, z, w; };   // (Psi_x, Psi_y, Psi_z, Theta)
+template <class T> struct S2 { T s, c; };          // (sin, cos) pair, ".x/.y" of the reference's double2
+
+template <class T> inline V3<T> operator+(const V3<T>& a, const V3<T>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> operator-(const V3<T>& a, const V3<T>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> operator-(const V3<T>& a) { return {-a.x, -a.y, -a.z}; }
+template <class T> inline V3<T> operator*(T s, const V3<T>& a) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline T dot(const V3<T>& a, const V3<T>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline V3<T> cross(const V3<T>& a, const V3<T>& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> inline T vlen(const V3<T>& a) { return sqrt(dot(a, a)); }
+template <class T> inline T vlen2(const V3<T>& a) { return dot(a, a); }
+// cuda_math.cuh:249-258: multiply by the reciprocal of the length
+template <class T> inline V3<T> normalize(const V3<T>& a) {
+    T inv = T(1.0) / vlen(a);
+    return {a.x * inv, a.y * inv, a.z * inv};
+}
+// operator/= on Point3 (cuda_math.cuh:237-241): three true divisions
+template <class T> inline V3<T> divv(const V3<T>& a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+
+template <class T> inline Q4<T> operator+(const Q4<T>& a, const Q4<T>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+template <class T> inline Q4<T> operator-(const Q4<T>& a, const Q4<T>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+template <class T> inline Q4<T> operator*(T s, const Q4<T>& a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+template <class T> inline Q4<T> vecpart(const V3<T>& v) { return {v.x, v.y, v.z, T(0.0)}; }
+template <class T> inline V3<T> vecof(const Q4<T>& q) { return {q.x, q.y, q.z}; }
+
+template <class T> inline T sqr(T x) { return x * x; }
+
+// cuda_math.cuh:31-36 -- sign with a DOUBLE_MIN dead band
+template <class T> inline double sign_db(T x) {
+    if (fabs(val(x)) < DOUBLE_MIN) return 0.0;
+    return (val(x) > DOUBLE_MIN) ? 1.0 : -1.0;
+}
+// cuda_math.cuh:47-49 -- arg x = 0 for x > DOUBLE_MIN, else pi
+template <class T> inline double arg_db(T x) { return (val(x) > DOUBLE_MIN) ? 0.0 : PI; }
+
+// cuda_math.cu:14-27 -- angle between two vectors via acos with clamps
+template <class T> inline T angle(const V3<T>& a, const V3<T>& b) {
+    T den = sqrt(vlen2(a) * vlen2(b));
+    if (val(den) < EPS_ZERO) return T(0.0);
+    T r = dot(a, b) / den;
+    if (val(r) >= 1.0) return T(0.0);
+    if (val(r) <= -1.0) return T(PI);
+    return acos(r);
+}
+
+// cuda_math.cuh:270-280 -- component-wise division with zero guards
+inline Q4<double> divide_guard(const Q4<double>& a, const Q4<double>& b) {
+    Q4<double> r;
+    r.x = (std::fabs(a.x) < DOUBLE_MIN && std::fabs(b.x) < DOUBLE_MIN) ? 0.0 : (a.x / b.x);
+    r.y = (std::fabs(a.y) < DOUBLE_MIN && std::fabs(b.y) < DOUBLE_MIN) ? 0.0 : (a.y / b.y);
+    r.z = (std::fabs(a.z) < DOUBLE_MIN && std::fabs(b.z) < DOUBLE_MIN) ? 0.0 : (a.z / b.z);
+    r.w = (std::fabs(a.w) < EPS_ZERO2 && std::fabs(b.w) < EPS_ZERO2) ? 0.0 : (a.w / b.w);
+    return r;
+}
+inline double norm1(const Q4<double>& a) { return std::fabs(a.x) + std::fabs(a.y) + std::fabs(a.z) + std::fabs(a.w); }
+inline double norm1(const V3<double>& a) { return std::fabs(a.x) + std::fabs(a.y) + std::fabs(a.z); }
+
+// evaluator3d.cu:76-99 -- Runge criterion; true when the task has NOT converged
+inline bool runge_unconverged(const Q4<double>& cur, const Q4<double>& prev, double pow2p, double tol, double* crit_out = nullptr) {
+    Q4<double> num = cur - prev;
+    Q4<double> den = pow2p * prev - cur;
+    double crit = norm1(divide_guard(num, den));
+    if (crit_out) *crit_out = crit;
+    return crit > tol;
+}
+
+// ---------------------------------------------------------------- topology helpers
+struct I3 { int x, y, z; };
+inline int comp(const I3& t, int k) { return k == 0 ? t.x : (k == 1 ? t.y : t.z); }
+
+// evaluatorJ3DK.cu:782-795 -- position of the shared vertex in each triangle
+// (the inner loop's break leaves the outer loop running, so the last match wins)
+inline void shifts_vertex(const I3& t1, const I3& t2, int* s1, int* s2) {
+    int a = 0, b = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (comp(t1, i) == comp(t2, j)) { a = i; b = j; break; }
+    *s1 = a; *s2 = b;
+}
+// evaluatorJ3DK.cu:797-825 -- position of the vertex NOT on the shared edge
+inline void shifts_edge(const I3& t1, const I3& t2, int* s1, int* s2) {
+    int ai[2] = {0, 0}, aj[2] = {0, 0}, n = 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (comp(t1, i) == comp(t2, j) && n < 2) { ai[n] = i; aj[n] = j; ++n; }
+    int r1 = 0, r2 = 0;
+    for (int i = 0; i < 3; ++i) if (ai[0] != i && ai[1] != i) { r1 = i; break; }
+    for (int i = 0; i < 3; ++i) if (aj[0] != i && aj[1] != i) { r2 = i; break; }
+    *s1 = r1; *s2 = r2;
+}
+// evaluatorJ3DK.cu:827-847 -- cyclic left rotation of the vertex triple
+inline I3 rotate_left(const I3& t, int shift) {
+    if (shift > 3) return {0, 0, 0};
+    if (shift == 3) return t;
+    int v[3];
+    for (int i = 0; i < 3; ++i) { int o = i + shift; if (o >= 3) o -= 3; v[i] = comp(t, o); }
+    return {v[0], v[1], v[2]};
+}
+
+// ---------------------------------------------------------------- inner integral
+// evaluatorJ3DK.cu:266-313 -- analytic (Psi, Theta) of panel (A,B,C) at point pt
+template <class T>
+Q4<T> theta_psi(const V3<T>& pt, const V3<T>& A, const V3<T>& B, const V3<T>& C) {
+    V3<T> ova = pt - A, ovb = pt - B, ovc = pt - C;
+    T lva = vlen(ova), lvb = vlen(ovb), lvc = vlen(ovc);
+    ova = divv(ova, lva); ovb = divv(ovb, lvb); ovc = divv(ovc, lvc);
+    const V3<T> taua = normalize(C - B), taub = normalize(A - C), tauc = normalize(B - A);
+    const T rac = dot(ova, tauc), rbc = dot(ovb, tauc), rba = dot(ovb, taua),
+            rca = dot(ovc, taua), rcb = dot(ovc, taub), rab = dot(ova, taub);
+    T t1, t2, t3;
+    if (fabs(val(rbc) + 1.0) < 0.5 * EPS_PSI_THETA2) t1 = log(lvb / lva);
+    else t1 = log((lva * (T(1.0) + rac)) / (lvb * (T(1.0) + rbc)));
+    if (fabs(val(rca) + 1.0) < 0.5 * EPS_PSI_THETA2) t2 = log(lvc / lvb);
+    else t2 = log((lvb * (T(1.0) + rba)) / (lvc * (T(1.0) + rca)));
+    if (fabs(val(rab) + 1.0) < 0.5 * EPS_PSI_THETA2) t3 = log(lva / lvc);
+    else t3 = log((lvc * (T(1.0) + rcb)) / (lva * (T(1.0) + rab)));
+    Q4<T> r = vecpart(t1 * tauc + t2 * taua + t3 * taub);
+    r.w = T(2.0) * atan2(dot(cross(ova, ovb), ovc), T(1.0) + dot(ova, ovb) + dot(ovb, ovc) + dot(ovc, ova));
+    return r;
+}
+
+// ---------------------------------------------------------------- singular parts (point functions)
+// evaluatorJ3DK.cu:315-350 -- singular behaviour near the shared edge.
+// JA is the vertex of K_j off the shared edge; (JB, JC) is the shared edge.
+template <class T>
+Q4<T> sing_edge(const V3<T>& pt, const V3<T>& JA, const V3<T>& JB, const V3<T>& JC) {
+    V3<T> va = pt - JB, vb = pt - JC;
+    const T lva = vlen(va), lvb = vlen(vb);
+    va = divv(va, lva); vb = divv(vb, lvb);
+    const V3<T> taua = normalize(JA - JC), taub = normalize(JB - JA);
+    V3<T> tauc = JC - JB;
+    const T ilvc = T(1.0) / vlen(tauc);
+    tauc = ilvc * tauc;
+    Q4<T> r = vecpart(log((lvb * dot(tauc, tauc - vb)) / (lva * dot(tauc, tauc - va))) * tauc -
+                      log(lva * dot(taub, taub + va) * ilvc) * taub -
+                      log(lvb * dot(taua, taua - vb) * ilvc) * taua);
+    r.w = T(2.0) * (atan2(dot(cross(va, taub), tauc), dot(taub - tauc, taub + va)) -
+                    atan2(dot(cross(vb, taua), tauc), dot(taua - tauc, taua - vb)));
+    return r;
+}
+
+// Per-pair invariants of the vertex-sharing singular part (evaluatorJ3DK.cu:356-401)
+template <class T> struct VertexCtx {
+    V3<T> JA, taua, taub, e;
+    T invAri;
+};
+// evaluatorJ3DK.cu:376-399 -- direction e along the intersection of the panel
+// planes with the two sign-flip corrections of the delta angles
+template <class T>
+V3<T> vertex_e(const V3<T>& nI, const V3<T>& nJ, const V3<T>& taua, const V3<T>& taub, T* dA = nullptr, T* dB = nullptr) {
+    V3<T> e = cross(nI, nJ);
+    if (val(vlen2(e)) < EPS_ZERO2) e = taub;
+    else e = normalize(e);
+    auto deltas = [&](const V3<T>& ee, T* x, T* y) {
+        *x = atan2(dot(cross(taua, ee), nJ), -dot(ee, taua));
+        *y = atan2(dot(cross(ee, taub), nJ), dot(ee, taub));
+    };
+    T dx, dy;
+    deltas(e, &dx, &dy);
+    if ((PI - fabs(val(dx)) < EPS_ZERO) || (PI - fabs(val(dy)) < EPS_ZERO)) { e = -e; deltas(e, &dx, &dy); }
+    if ((val(dx) * val(dy) < 0) && (fabs(val(dx) - val(dy)) > PI)) { e = -e; deltas(e, &dx, &dy); }
+    if (dA) *dA = dx;
+    if (dB) *dB = dy;
+    return e;
+}
+template <class T>
+VertexCtx<T> vertex_ctx(const V3<T>& JA, const V3<T>& JB, const V3<T>& JC, const V3<T>& nI, const V3<T>& nJ, T areaI) {
+    VertexCtx<T> c;
+    c.JA = JA;
+    c.taua = normalize(JA - JC);
+    c.taub = normalize(JB - JA);
+    c.e = vertex_e(nI, nJ, c.taua, c.taub);
+    c.invAri = T(1.0) / sqrt(areaI);   // rsqrt(measures[i]) at evaluatorJ3DK.cu:401
+    return c;
+}
+// evaluatorJ3DK.cu:352-407 -- singular behaviour near the shared vertex JA
+template <class T>
+Q4<T> sing_vertex(const V3<T>& pt, const VertexCtx<T>& c) {
+    V3<T> ovc = pt - c.JA;
+    const T lvc = vlen(ovc);
+    ovc = divv(ovc, lvc);
+    Q4<T> r = vecpart(-(log((lvc * (T(1.0) + dot(c.taua, ovc))) * c.invAri) * c.taua +
+                        log((lvc * (T(1.0) - dot(c.taub, ovc))) * c.invAri) * c.taub));
+    r.w = T(2.0) * (atan2(dot(cross(ovc, c.taua), c.e), dot(c.e - ovc, c.e - c.taua)) +
+                    atan2(dot(cross(ovc, c.taub), c.e), dot(c.e - ovc, c.e + c.taub)));
+    return r;
+}
+
+// ---------------------------------------------------------------- analytic singular integrals
+// evaluatorJ3DK.cu:418-421
+template <class T> T phi_f(S2<T> a, S2<T> g, T sinXi, T cosLambda) {
+    return T(2.0) * atan2(sinXi * a.s * g.s, T(1.0) - a.c + g.c + cosLambda);
+}
+// evaluatorJ3DK.cu:438-461 -- (q_Theta, q_Psi), general case.  Pinned by cubature.
+template <class T>
+S2<T> q_theta_psi(S2<T> A, S2<T> B, S2<T> G, S2<T> Nu, S2<T> Xi, T cosMu, T cosLambda) {
+    const T phi1 = phi_f(A, G, Xi.s, cosLambda);
+    const T phi2 = phi_f(S2<T>{A.s, -A.c}, S2<T>{G.s, -G.c}, Xi.s, cosLambda);
+    const T iS = T(1.0) / (A.s * (T(1.0) - cosMu * cosMu));
+    S2<T> r;
+    r.s = phi1 + G.s * Nu.s * iS * (
+              (B.c * G.s - Xi.c * B.s * G.c) * phi2 +
+              Xi.s * B.s * (T(0.5) * (T(1.0) + cosMu) * log((T(1.0) + B.c) / (T(1.0) - Nu.c)) +
+                            T(0.5) * (T(1.0) - cosMu) * log((T(1.0) - B.c) / (T(1.0) + Nu.c)) +
+                            log((T(1.0) + cosLambda) / (T(1.0) - G.c))));
+    r.c = T(1.5) - iS * (
+              B.s * (Nu.c + cosMu * cosLambda) * log(T(1.0) + cosLambda) +
+              Nu.s * (B.c + cosMu * G.c) * log(T(1.0) - G.c) +
+              B.s * (T(1.0) - cosMu) * (Nu.c - cosLambda) * log(B.s / Nu.s) +
+              Nu.s * B.s * (B.s * G.c - Xi.c * G.s * B.c) * log((T(1.0) - Nu.c) / (T(1.0) + B.c)) +
+              phi2 * Xi.s * G.s * Nu.s * B.s);
+    return r;   // .s = q_Theta, .c = q_Psi
+}
+// evaluatorJ3DK.cu:474-481 -- coplanar case.  The reference leaves q_Theta
+// unset; its documented value (evaluatorJ3DK.cu:471) is 0, used here.
+template <class T>
+S2<T> q_theta_psi_zero(S2<T> B, S2<T> Nu, T sinAlpha) {
+    S2<T> r;
+    r.s = T(0.0);
+    r.c = T(1.5) - (Nu.c * B.s * log(T(1.0) + Nu.c) + Nu.s * B.c * log(T(1.0) - B.c) + sinAlpha
+                    - B.s + Nu.s + B.s * Nu.c * log(B.s / Nu.s)) / sinAlpha;
+    return r;
+}
+
+// evaluatorJ3DK.cu:511-601 -- (q^Theta, q^Psi) for vertex-sharing pairs with
+// its special cases.  Pinned by cubature.
+template <class T>
+S2<T> q_theta_psi_cont(S2<T> Xi, T mu, S2<T> Mu, T logSinMu, T logSinNu, T psi, S2<T> Psi, T nu, S2<T> Nu,
+                       T kappa, T sinKappa, T sinNuPsi, T sinMuPsi, T delta, S2<T> Delta,
+                       T cosLambda, T cosTheta, T cosEta, T cosSigma, T cosChi) {
+    const double LOG2 = std::log(2.0);
+    S2<T> r;
+    const T logOneCosTheta = log(T(1.0) + cosTheta);
+    const T logOneCosLambda = log(T(1.0) + cosLambda);
+    const T Lambda1 = logOneCosLambda - logOneCosTheta + logSinNu - logSinMu;
+    const T Lambda2 = log(tan(T(0.5) * nu) * tan(T(0.5) * mu));
+
+    S2<T> h05D; sincos_t(T(0.5) * delta, &h05D.s, &h05D.c);
+    const T tan05Delta = h05D.s / h05D.c;
+    S2<T> h05MuPsi, h05NuPsi;
+    sincos_t(T(0.5) * (mu - psi), &h05MuPsi.s, &h05MuPsi.c);
+    sincos_t(T(0.5) * (nu + psi), &h05NuPsi.s, &h05NuPsi.c);
+    const T Amu = atan2(tan05Delta * h05MuPsi.c * Xi.s, tan05Delta * h05MuPsi.c * Xi.c + h05MuPsi.s);
+    const T Anu = atan2(tan05Delta * h05NuPsi.s * Xi.s, tan05Delta * h05NuPsi.s * Xi.c + h05NuPsi.c);
+
+    S2<T> tmp, tmp2;
+    sincos_t(T(0.5) * (mu - psi) - T(0.5) * (nu + psi), &tmp.s, &tmp.c);
+    sincos_t(T(0.5) * kappa, &tmp2.s, &tmp2.c);
+    const T W = atan2(Delta.s * tmp2.s * Xi.s, tmp2.c + Delta.s * tmp.c * Xi.c + Delta.c * tmp.s);
+
+    sincos_t(delta - psi, &tmp.s, &tmp.c);
+    const T D = T(1.0) / (sqr(tmp.s) + Delta.s * Psi.s * (T(1.0) - Xi.c) * (tmp.c + cosSigma));
+    const T G = Psi.c * (Delta.s * cosSigma * Xi.c + Delta.c * cosChi - sqr(Delta.s) / Psi.s);
+
+    const T gent = T(2.0) * (Anu * Mu.s * sinNuPsi - Amu * Nu.s * sinMuPsi -
+                             D * Mu.s * Nu.s * Delta.s * (W * cosEta + T(0.5) * Psi.s * Xi.s * (Lambda1 - Lambda2 * cosSigma))) /
+                   (Psi.s * sinKappa);
+    const T gens = T(0.5 * (3.0 - LOG2)) +
+                   (Mu.s * Nu.s / sinKappa) * ((logOneCosLambda - logOneCosTheta) * Psi.c / Psi.s +
+                                              D * (Lambda1 * Delta.s * cosEta / Psi.s + Lambda2 * cosChi -
+                                                   T(2.0) * W * Delta.s * Xi.s - G * (logSinNu - logSinMu))) -
+                   (Mu.c * Nu.s * (logOneCosLambda - logSinMu) + Mu.s * Nu.c * (logOneCosTheta - logSinNu)) / sinKappa -
+                   T(0.5) * (logSinMu + logSinNu - log(sinKappa));
+
+    const double mulPsi = sign_db(Xi.s * Psi.s);
+    const double mulDelta = sign_db(Xi.s * Delta.s);
+
+    if ((fabs(val(Xi.s)) < EPS_ZERO) && (1.0 - fabs(val(cosSigma)) < 0.5 * EPS_ZERO2) && (fabs(val(Psi.s)) > EPS_ZERO)) {
+        const double ara = arg_db(sin(T(0.5) * (nu + psi)) * cosSigma);
+        const double arb = arg_db(cos(T(0.5) * (mu - psi)) * cosSigma);
+        r.s = T(2.0 * mulPsi) * cosSigma * Xi.c / (sinKappa * Psi.s) * (Mu.s * sinNuPsi * T(ara) - Nu.s * sinMuPsi * T(arb));
+        r.c = T(0.5 * (1.0 - LOG2)) - T(0.5) * log((T(1.0) - cosSigma * Mu.c) * (T(1.0) + cosSigma * Nu.c) / sinKappa) +
+              cosSigma * (Mu.s - Nu.s + T(0.5) * sin(mu - nu) * Lambda2) / sinKappa;
+        return r;
+    }
+    if ((fabs(val(Xi.s)) < EPS_ZERO) && (fabs(val(Psi.s)) > EPS_ZERO)) {
+        const double are = arg_db(sin(T(0.5) * (nu + mu)) + sin(T(0.5) * (mu - nu) - psi + delta * Xi.c));
+        const double arc = arg_db(T(1.0) / tan(T(0.5) * (nu + psi)) + tan(T(0.5) * delta) * Xi.c);
+        const double ard = arg_db(tan(T(0.5) * (mu - psi)) + tan(T(0.5) * delta) * Xi.c);
+        r.s = T(2.0 * mulDelta) * (Delta.s * Mu.s * Nu.s / cosChi * Xi.c * T(are) +
+                                   Mu.s * sinNuPsi * T(arc) - Nu.s * sinMuPsi * T(ard)) / (sinKappa * Psi.s);
+        r.c = gens;
+        return r;
+    }
+    if (fabs(val(sin(psi))) < EPS_ZERO) {
+        if (fabs(val(delta)) > EPS_ZERO) {
+            r.s = T(2.0) * (Mu.s * Nu.s * (W * Delta.c * Xi.c - T(0.5) * (Lambda1 - Lambda2 * cosSigma) * Xi.s) / Delta.s +
+                            (Anu + T(mulDelta * arg_db(sin(T(0.5) * (nu + psi))))) * Mu.s * Nu.c +
+                            (Amu + T(mulDelta * arg_db(cos(T(0.5) * (mu - psi))))) * Mu.c * Nu.s) / sinKappa;
+            r.c = T(0.5 * (3.0 - LOG2)) - T(0.5) * (log((T(1.0) + cosLambda) * (T(1.0) + cosTheta) / sinKappa) - sin(mu - nu) / sinKappa * Lambda1) -
+                  Mu.s * Nu.s * ((Lambda1 * Delta.c - Lambda2 * Psi.c) * Xi.c + T(2.0) * W * Xi.s) / (sinKappa * Delta.s);
+            return r;
+        } else {
+            r.s = T(2.0 * arg_db(Psi.c));
+            r.c = T(0.5 * (1.0 - LOG2)) - T(0.5) * log((T(1.0) - Psi.c * Mu.c) * (T(1.0) + Psi.c * Nu.c) / sinKappa) +
+                  (T(0.5) * sin(mu - nu) * Lambda1 + Psi.c * (Mu.s - Nu.s)) / sinKappa;
+            return r;
+        }
+    }
+    r.s = gent;
+    r.c = gens;
+    return r;
+}
+
+// Vertices of a pair rotated as the reference does before the analytic formulas.
+template <class T> struct PairVerts { V3<T> IA, IB, IC, JA, JB, JC; };
+
+// evaluatorJ3DK.cu:603-672 -- closed-form integral over K_i of the edge-sharing
+// singular part.  Pinned by cubature.
+template <class T>
+Q4<T> integrate_sing_edge(const PairVerts<T>& p, const V3<T>& nI, const V3<T>& nJ, T areaI) {
+    const V3<T> taua = normalize(p.JA - p.JC), taub = normalize(p.JB - p.JA), tauc = normalize(p.JC - p.JB);
+    const T alpha = angle(p.IA - p.IC, p.IB - p.IC);
+    const T beta = angle(p.IC - p.IB, p.IA - p.IB);
+    const T gamma = angle(p.JC - p.JB, p.JA - p.JB);
+    const T delta = angle(p.JB - p.JC, p.JA - p.JC);
+    const T nu = T(PI) - alpha - beta;
+    const T xi = atan2(dot(cross(nI, nJ), tauc), dot(nI, nJ));
+    S2<T> A, B, G, D, X, N;
+    sincos_t(alpha, &A.s, &A.c); sincos_t(beta, &B.s, &B.c); sincos_t(gamma, &G.s, &G.c);
+    sincos_t(delta, &D.s, &D.c); sincos_t(xi, &X.s, &X.c);
+    N.s = A.s * B.c + A.c * B.s;
+    N.c = A.s * B.s - A.c * B.c;
+    const T cosSigma = -(A.c * D.c + X.c * A.s * D.s);
+    const T cosMu = -(B.c * G.c + X.c * B.s * G.s);
+    const T cosLambda = -(A.c * G.c - X.c * A.s * G.s);
+    const T cosTheta = -(B.c * D.c - X.c * B.s * D.s);
+    const T qab = N.s * log(tan(T(0.5) * alpha) * tan(T(0.5) * nu)) / B.s +
+                  N.s * log(tan(T(0.5) * beta) * tan(T(0.5) * nu)) / A.s +
+                  log(tan(T(0.5) * alpha) * tan(T(0.5) * beta));
+    S2<T> qA, qB;
+    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(beta) - val(gamma)) < EPS_ZERO)) qA = q_theta_psi_zero(B, N, A.s);
+    else qA = q_theta_psi(A, B, G, N, X, cosMu, cosLambda);
+    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(alpha) - val(delta)) < EPS_ZERO)) qB = q_theta_psi_zero(A, N, B.s);
+    else qB = q_theta_psi(B, A, D, N, X, cosSigma, cosTheta);
+    Q4<T> r = vecpart(areaI * (qA.c * taub + qB.c * taua - qab * tauc));
+    r.w = areaI * (qA.s + qB.s);
+    return r;
+}
+
+// evaluatorJ3DK.cu:674-780 -- closed-form integral over K_i of the
+// vertex-sharing singular part.  Pinned by cubature.
+template <class T>
+Q4<T> integrate_sing_vertex(const PairVerts<T>& p, const V3<T>& nI, const V3<T>& nJ, T areaI) {
+    const V3<T> taua = normalize(p.JA - p.JC), taub = normalize(p.JB - p.JA);
+    T dA, dB;
+    const V3<T> e = vertex_e(nI, nJ, taua, taub, &dA, &dB);
+    const T xi = atan2(dot(cross(nI, nJ), e), dot(nI, nJ));
+    S2<T> DA, DB, X;
+    sincos_t(dA, &DA.s, &DA.c); sincos_t(dB, &DB.s, &DB.c); sincos_t(xi, &X.s, &X.c);
+    const V3<T> s = p.IC - p.IB;
+    const T nu = angle(p.IA - p.IB, p.IC - p.IB);
+    const T mu = angle(p.IB - p.IC, p.IA - p.IC);
+    const T kappa = angle(p.IB - p.IA, p.IC - p.IA);
+    S2<T> Mu, Nu;
+    sincos_t(mu, &Mu.s, &Mu.c); sincos_t(nu, &Nu.s, &Nu.c);
+    const T logSinNu = log(Nu.s), logSinMu = log(Mu.s);
+    const T sinKappa = sin(kappa);
+    const T psi = atan2(dot(cross(e, s), nI), dot(e, s));
+    S2<T> Ps; sincos_t(psi, &Ps.s, &Ps.c);
+    S2<T> NuPsi, MuPsi;
+    sincos_t(nu + psi, &NuPsi.s, &NuPsi.c);
+    sincos_t(mu - psi, &MuPsi.s, &MuPsi.c);
+    auto cosines = [&](S2<T> Dl, T* cSig, T* cChi, T* cEta, T* cTh, T* cLam) {
+        *cSig = Dl.s * Ps.s * X.c + Dl.c * Ps.c;
+        *cChi = Dl.s * Ps.c * X.c - Dl.c * Ps.s;
+        *cEta = Dl.c * Ps.s * X.c - Dl.s * Ps.c;
+        *cTh = Dl.s * NuPsi.s * X.c + Dl.c * NuPsi.c;
+        *cLam = Dl.s * MuPsi.s * X.c - Dl.c * MuPsi.c;
+    };
+    T sA, cA_, eA, tA, lA, sB, cB_, eB, tB, lB;
+    cosines(DA, &sA, &cA_, &eA, &tA, &lA);
+    cosines(DB, &sB, &cB_, &eB, &tB, &lB);
+    const S2<T> qA = q_theta_psi_cont(X, mu, Mu, logSinMu, logSinNu, psi, Ps, nu, Nu, kappa, sinKappa,
+                                      NuPsi.s, MuPsi.s, dA, DA, lA, tA, eA, sA, cA_);
+    const S2<T> qB = q_theta_psi_cont(X, mu, Mu, logSinMu, logSinNu, psi, Ps, nu, Nu, kappa, sinKappa,
+                                      NuPsi.s, MuPsi.s, dB, DB, lB, tB, eB, sB, cB_);
+    Q4<T> r = vecpart(areaI * (qA.c * taua + qB.c * taub));
+    r.w = (fabs(val(xi)) < EPS_ZERO) ? T(0.0) : (areaI * (qA.s - qB.s));
+    return r;
+}
+
+// ---------------------------------------------------------------- finalize
+// evaluatorJ3DK.cu:224-241 -- vertex-sharing: 2*pi*p wrap of Theta, then J
+template <class T>
+V3<T> finalize_vertex(const Q4<T>& I, const V3<T>& nJ, T areaI, int* p_out = nullptr) {
+    int p = 0;
+    const double refTheta = TWO_PI * val(areaI);
+    if (val(I.w) > refTheta) p = -((int)std::trunc((val(I.w) - refTheta) / (2.0 * refTheta)) + 1);
+    else if (val(I.w) < -refTheta) p = ((int)std::trunc((-refTheta - val(I.w)) / (2.0 * refTheta)) + 1);
+    if (p_out) *p_out = p;
+    const T refT = T(TWO_PI) * areaI;
+    return T(RECIPROCAL_FOUR_PI) * ((I.w + T(2.0 * p) * refT) * nJ + cross(vecof(I), nJ));
+}
+// evaluatorJ3DK.cu:254-264 -- edge-sharing and disjoint: J = (Theta n_j + Psi x n_j)/(4 pi)
+template <class T>
+V3<T> finalize_plain(const Q4<T>& I, const V3<T>& nJ) {
+    return T(RECIPROCAL_FOUR_PI) * (I.w * nJ + cross(vecof(I), nJ));
+}
+
+// ---------------------------------------------------------------- quadrature
+// QuadratureFormula3D (QuadratureFormula3d.cuh:20-31) uploaded with
+// L3 = 1 - L1 - L2 (NumericalIntegrator3d.cu:200-206)
+struct Rule {
+    int n = 0, order = 0;
+    double L[13][3];
+    double w[13];
+};
+inline Rule make_rule(const double* L12, const double* w, int n, int order) {
+    Rule r; r.n = n; r.order = order;
+    for (int k = 0; k < n; ++k) {
+        r.L[k][0] = L12[2 * k]; r.L[k][1] = L12[2 * k + 1];
+        r.L[k][2] = 1.0 - L12[2 * k] - L12[2 * k + 1];
+        r.w[k] = w[k];
+    }
+    return r;
+}
+// NumericalIntegrator3d.cu:569-585 -- Gauss point from the panel's vertices
+template <class T>
+V3<T> gauss_point(const Rule& R, int k, const V3<T>& v0, const V3<T>& v1, const V3<T>& v2) {
+    V3<T> r = T(R.L[k][0]) * v0;
+    r = r + T(R.L[k][1]) * v1;
+    r = r + T(R.L[k][2]) * v2;
+    return r;
+}
+
+// A refined sub-panel: vertex positions and area.  The split of
+// NumericalIntegrator3d.cu:54-72: midpoints m_BC, m_CA, m_AB and children
+// {m_AB, B, m_BC}, {m_BC, C, m_CA}, {m_CA, A, m_AB}, {m_BC, m_CA, m_AB},
+// each with a quarter of the parent's area.
+template <class T> struct SubPanel { V3<T> a, b, c; T area; };
+template <class T>
+SubPanel<T> split_child(const SubPanel<T>& P, int child) {
+    const V3<T> mBC = T(0.5) * (P.b + P.c);
+    const V3<T> mCA = T(0.5) * (P.c + P.a);
+    const V3<T> mAB = T(0.5) * (P.a + P.b);
+    const T q = T(0.25) * P.area;
+    switch (child) {
+        case 0: return {mAB, P.b, mBC, q};
+        case 1: return {mBC, P.c, mCA, q};
+        case 2: return {mCA, P.a, mAB, q};
+        default: return {mBC, mCA, mAB, q};
+    }
+}
+
+// Pair evaluation context for one ordered task (i, j) of a given type.
+enum PairType { VERTEX_SHARING = 0, EDGE_SHARING = 1, DISJOINT = 2 };
+
+template <class T> struct PairEval {
+    int type;
+    V3<T> A, B, C;            // original K_j (inner integral)
+    V3<T> eJA, eJB, eJC;      // rotated K_j for the edge-sharing singular part
+    VertexCtx<T> vc;          // vertex-sharing singular part invariants
+    // integrand of the numerical part at pt (thetaPsi minus the singular part)
+    Q4<T> f(const V3<T>& pt) const {
+        Q4<T> tp = theta_psi(pt, A, B, C);
+        if (type == EDGE_SHARING) return tp - sing_edge(pt, eJA, eJB, eJC);
+        if (type == VERTEX_SHARING) return tp - sing_vertex(pt, vc);
+        return tp;
+    }
+};
+
+// NumericalIntegrator3d.cu:560-567 and evaluatorJ3DK.cu:98-106: area * sum_k w_k f(r_k)
+template <class T>
+Q4<T> quad_panel(const Rule& R, const SubPanel<T>& P, const PairEval<T>& E) {
+    Q4<T> acc{T(0.0), T(0.0), T(0.0), T(0.0)};
+    for (int k = 0; k < R.n; ++k) {
+        Q4<T> fk = E.f(gauss_point(R, k, P.a, P.b, P.c));
+        acc = acc + T(R.w[k]) * fk;
+    }
+    return P.area * acc;
+}
+// Sum over the 4^level descendants of P (depth-first child order).  The
+// reference gathers the same terms with atomicAdd in arbitrary order
+// (NumericalIntegrator3d.cu:160-173).
+template <class T>
+Q4<T> level_value(const Rule& R, const SubPanel<T>& P, int level, const PairEval<T>& E) {
+    if (level == 0) return quad_panel(R, P, E);
+    Q4<T> s{T(0.0), T(0.0), T(0.0), T(0.0)};
+    for (int c = 0; c < 4; ++c) s = s + level_value(R, split_child(P, c), level - 1, E);
+    return s;
+}
+
+}  // namespace i2o
