@@ -41,11 +41,17 @@ extern "C" {
 typedef struct i2_mesh i2_mesh;             /* Mesh3D */
 typedef struct i2_integrator i2_integrator; /* NumericalIntegrator3D */
 typedef struct i2_evaluator i2_evaluator;   /* Evaluator3D / EvaluatorJ3DK */
+typedef struct i2_batch i2_batch;           /* batched path (no counterpart in the reference) */
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* i2_last_error(void);
 /* Number of visible HIP devices (0 without a GPU). */
 int i2_device_count(int* n);
+/* Host-side evaluation of the inner integral (Psi, Theta) of triangle tri9 at
+ * n points, with the device formulas compiled for the CPU: fast = 0 is the
+ * reference's expression (thetaPsi, src/evaluators/evaluatorJ3DK.cu:266-313),
+ * fast = 1 the division-free form used by the batched kernels.  For checking. */
+int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9, double* out4);
 
 /* ======================================================================
  * Mesh3D (src/Mesh3d.cuh:55-126, src/Mesh3d.cu)
@@ -120,10 +126,56 @@ int i2_evaluator_refinement_iterations(const i2_evaluator* ev, int type, int* K)
 /* Device time (ms) of the last run for a type and the number of refined
  * (sub-panel, panel) tasks integrated. */
 int i2_evaluator_type_stats(const i2_evaluator* ev, int type, double* ms, int64_t* refinedTasks);
+/* Profiling mode: record HIP events around every integration launch
+ * (GpuTimer of src/common/gpu_timer.cuh:14-77 times whole types only). */
+int i2_evaluator_set_profiling(i2_evaluator* ev, int enable);
+/* Launch records of the last run: (type, level) pairs, task counts and device
+ * milliseconds; *count receives the number of records (capacity may be 0). */
+int i2_evaluator_launch_records(const i2_evaluator* ev, int64_t capacity, int32_t* type_level, int64_t* tasks,
+                                double* ms, int64_t* count);
+/* hipDeviceSynchronize on `device`. */
+int i2_device_synchronize(int device);
 /* Evaluator3D::outputResultsToFile (src/evaluators/evaluator3d.cu:354-457):
  * writes SimpleNeighbors/AttachedNeighbors/NotNeighbors .dat or .csv into dir
  * (NULL = current directory). */
 int i2_evaluator_output_results_to_file(const i2_evaluator* ev, int type, int format, const char* dir);
+
+/* Integration path of runAllPairs / runPairs: 0 = plain per-type, per-level
+ * launches (the reference's structure, src/evaluators/evaluatorJ3DK.cu:895-1012);
+ * 1 = the batched persistent kernels (runAllPairs only).  Both produce the same
+ * outputs in the same task order. */
+int i2_evaluator_set_path(i2_evaluator* ev, int path);
+/* Launch shape of the batched path: pairs per workgroup and quadrature points
+ * per lane (1, 2, 4 or 13 = the whole 13-point rule in one lane). */
+int i2_evaluator_set_tuning(i2_evaluator* ev, int pairsPerWorkgroup, int pointsPerLane);
+
+/* ======================================================================
+ * Batched path: many meshes, all neighbour types and refinement levels in one
+ * captured HIP graph of persistent kernels fed by device work queues (no
+ * counterpart in the reference, which runs one mesh type by type).
+ * ==================================================================== */
+#define I2_BATCH_PER_PAIR 0  /* per-pair integrals and J, reference task order */
+#define I2_BATCH_ROW_SUMS 1  /* per control panel i: sum over j of (J, Theta) */
+int i2_batch_create(i2_batch** out, int device);
+void i2_batch_destroy(i2_batch* b);
+/* Adds a prepared mesh (same device); meshes are borrowed and must outlive the batch. */
+int i2_batch_add_mesh(i2_batch* b, const i2_mesh* m, int* index);
+int i2_batch_clear(i2_batch* b);
+/* Rule and error control are taken from an integrator object. */
+int i2_batch_configure(i2_batch* b, const i2_integrator* it, int outputMode);
+/* Launch shape (see i2_evaluator_set_tuning). */
+int i2_batch_set_tuning(i2_batch* b, int pairsPerWorkgroup, int pointsPerLane);
+/* Runs every task of every mesh; the first run after a change captures the
+ * graph, later runs replay it.  ms receives the device time of the run. */
+int i2_batch_run(i2_batch* b, double* ms);
+int i2_batch_get_pair_results(const i2_batch* b, int mesh, int type, double* integrals4, double* results3, int8_t* conv);
+/* rows4[i] = (sum_j J_ij, sum_j Theta_ij) over all three types. */
+int i2_batch_get_row_sums(const i2_batch* b, int mesh, double* rows4);
+int i2_batch_refinement_iterations(const i2_batch* b, int mesh, int type, int* K);
+/* counts[3*L + t]: tasks of type t integrated at level L in the last run. */
+int i2_batch_level_counts(const i2_batch* b, int64_t* counts);
+/* Total ordered tasks of the batch. */
+int i2_batch_task_count(const i2_batch* b, int64_t* n);
 
 /* ======================================================================
  * Synthetic meshes (no counterpart in the reference, which loads files)

@@ -76,6 +76,10 @@ def _check(status):
         raise I2Error(f"status {status}: {msg}")
 
 
+def synchronize(device=0):
+    _check(lib().i2_device_synchronize(ctypes.c_int(device)))
+
+
 def device_count():
     n = ctypes.c_int(0)
     _check(lib().i2_device_count(ctypes.byref(n)))
@@ -256,7 +260,93 @@ class EvaluatorJ3DK:
         _check(lib().i2_evaluator_type_stats(self._h, ctypes.c_int(t), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def setPath(self, path):
+        """0: plain per-type, per-level launches; 1: batched persistent kernels."""
+        _check(lib().i2_evaluator_set_path(self._h, ctypes.c_int(path)))
+
+    def setTuning(self, pairsPerWorkgroup=64, pointsPerLane=13):
+        _check(lib().i2_evaluator_set_tuning(self._h, ctypes.c_int(pairsPerWorkgroup), ctypes.c_int(pointsPerLane)))
+
+    def setProfiling(self, enable=True):
+        _check(lib().i2_evaluator_set_profiling(self._h, ctypes.c_int(int(enable))))
+
+    def launchRecords(self):
+        """List of (type, level, tasks, ms) of the last run's integration launches (profiling mode)."""
+        cnt = _I64()
+        _check(lib().i2_evaluator_launch_records(self._h, _I64(0), None, None, None, ctypes.byref(cnt)))
+        n = cnt.value
+        tl = np.zeros((n, 2), np.int32)
+        tasks = np.zeros(n, np.int64)
+        ms = np.zeros(n)
+        _check(lib().i2_evaluator_launch_records(self._h, _I64(n), _ptr(tl), _ptr(tasks), _ptr(ms), ctypes.byref(cnt)))
+        return [(int(tl[q, 0]), int(tl[q, 1]), int(tasks[q]), float(ms[q])) for q in range(n)]
+
     def outputResultsToFile(self, t, fmt, directory=None):
         st = lib().i2_evaluator_output_results_to_file(self._h, ctypes.c_int(t), ctypes.c_int(fmt),
                                                        None if directory is None else directory.encode())
         return st == 0
+
+
+# ---------------------------------------------------------------- batched path
+BATCH_PER_PAIR = 0
+BATCH_ROW_SUMS = 1
+
+
+class Batch:
+    """Many meshes in one captured graph of persistent kernels (i2_batch_*)."""
+
+    def __init__(self, device=0):
+        self._h = _P()
+        _check(lib().i2_batch_create(ctypes.byref(self._h), ctypes.c_int(device)))
+        self.meshes = []
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.i2_batch_destroy(self._h)
+            self._h = None
+
+    def addMesh(self, mesh):
+        idx = ctypes.c_int()
+        _check(lib().i2_batch_add_mesh(self._h, mesh._h, ctypes.byref(idx)))
+        self.meshes.append(mesh)
+        return idx.value
+
+    def configure(self, integrator, outputMode=BATCH_PER_PAIR):
+        self._integrator = integrator
+        _check(lib().i2_batch_configure(self._h, integrator._h, ctypes.c_int(outputMode)))
+
+    def setTuning(self, pairsPerWorkgroup=64, pointsPerLane=13):
+        _check(lib().i2_batch_set_tuning(self._h, ctypes.c_int(pairsPerWorkgroup), ctypes.c_int(pointsPerLane)))
+
+    def run(self):
+        ms = _D()
+        _check(lib().i2_batch_run(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def pairResults(self, mesh, t):
+        n = 2 * self.meshes[mesh].sizes()[2][t]
+        ints = np.zeros((n, 4))
+        res = np.zeros((n, 3))
+        conv = np.zeros(n, np.int8)
+        _check(lib().i2_batch_get_pair_results(self._h, ctypes.c_int(mesh), ctypes.c_int(t), _ptr(ints), _ptr(res), _ptr(conv)))
+        return dict(integrals=ints, results=res, conv=conv)
+
+    def rowSums(self, mesh):
+        out = np.zeros((self.meshes[mesh].sizes()[1], 4))
+        _check(lib().i2_batch_get_row_sums(self._h, ctypes.c_int(mesh), _ptr(out)))
+        return out
+
+    def refinementIterations(self, mesh, t):
+        K = ctypes.c_int()
+        _check(lib().i2_batch_refinement_iterations(self._h, ctypes.c_int(mesh), ctypes.c_int(t), ctypes.byref(K)))
+        return K.value
+
+    def levelCounts(self):
+        c = np.zeros((13, 3), np.int64)
+        _check(lib().i2_batch_level_counts(self._h, _ptr(c)))
+        return c
+
+    def taskCount(self):
+        n = _I64()
+        _check(lib().i2_batch_task_count(self._h, ctypes.byref(n)))
+        return n.value

@@ -1,0 +1,1020 @@
+// i2_batch.hip -- the batched persistent path: many meshes, all three
+// neighbour types and every refinement level in one captured HIP graph of
+// persistent kernels fed by device work queues.
+//
+// Semantics are those of EvaluatorJ3DK::numericalIntegration
+// (src/evaluators/evaluatorJ3DK.cu:895-1012) applied per mesh and type: level
+// 0, then iterations 1..maxLevel comparing consecutive levels by the Runge rule
+// (src/evaluators/evaluator3d.cu:76-99).  A task's levels depend only on the
+// task itself, so each task walks its own refinement levels; the only global
+// quantity, the number of iterations K of a (mesh, type), is derived on the
+// device from the levels the queues reached, and the reference's swapped
+// result buffers (src/evaluators/evaluatorJ3DK.cu:976-990) are reproduced from
+// each task's last two level values in the finalize kernel.
+//
+// Kernels (one graph):
+//   k_sweep    levels 0 and 1 of every task.  Disjoint pairs are enumerated
+//              implicitly: a workgroup item is one control panel i and P
+//              consecutive panels j (i is uniform, its panel record is read
+//              with scalar loads); lane groups of LPP lanes share a pair and
+//              split its quadrature points.  Vertex- and edge-sharing tasks
+//              come from the mesh's lists, P per item.  Items are claimed from
+//              an atomic counter by persistent workgroups.
+//   k_deep     refinement level L >= 2 of the tasks queued by level L-1:
+//              min(4^L, 64) lanes per task, one sub-panel per lane,
+//              fixed-order butterfly reduction.
+//   k_levels / k_compute_K   K per (mesh, type) from queue membership.
+//   k_finalize_pairs / k_finalize_rows   parity selection, J, outputs.
+#include "i2_batch.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace i2 {
+using namespace i2d;
+
+// ------------------------------------------------------------ device helpers
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+// position of the disjoint pair (i, j), i < j, in the lexicographic list
+__device__ __forceinline__ int64_t disjoint_rank(const MeshDesc& M, int i, int j) {
+    int64_t below = 0;
+    for (int64_t q = M.nbr_start[i]; q < M.nbr_start[i + 1]; ++q) {
+        if (M.nbr[q] < j) ++below; else break;
+    }
+    return M.row_start[i] + (j - i - 1) - below;
+}
+// task index of ordered disjoint pair (i, j) in [(i<j) list] ++ [reversed]
+__device__ __forceinline__ int64_t disjoint_task(const MeshDesc& M, int i, int j) {
+    return (i < j) ? disjoint_rank(M, i, j) : M.npairs[I2_NOT_NEIGHBORS] + disjoint_rank(M, j, i);
+}
+
+__device__ __forceinline__ int find_base(const int64_t* base, int n, int64_t x) {
+    int lo = 0, hi = n - 1;   // largest b < n with base[b] <= x
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ JGeom jgeom(const Panel& P) { return {P.a, P.b, P.c, P.ta, P.tb, P.tc}; }
+
+template <int TYPE> struct Sing;
+template <> struct Sing<I2_NOT_NEIGHBORS> {
+    __device__ __forceinline__ void init(const Panel&, const Panel&) {}
+    __device__ __forceinline__ q4 at(d3) const { return q4zero(); }
+};
+template <> struct Sing<I2_ATTACHED_NEIGHBORS> {
+    EdgeCtx c;
+    __device__ __forceinline__ void init(const Panel& I, const Panel& J) {
+        int s1, s2;
+        shifts_edge(I, J, &s1, &s2);
+        c = edge_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3));
+    }
+    __device__ __forceinline__ q4 at(d3 p) const { return sing_edge(p, c); }
+};
+template <> struct Sing<I2_SIMPLE_NEIGHBORS> {
+    VertexCtx c;
+    __device__ __forceinline__ void init(const Panel& I, const Panel& J) {
+        int s1, s2;
+        shifts_vertex(I, J, &s1, &s2);
+        c = vertex_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    }
+    __device__ __forceinline__ q4 at(d3 p) const { return sing_vertex(p, c); }
+};
+
+// closed-form singular integral of a task (evaluatorJ3DK.cu:603-780)
+__device__ __noinline__ q4 singular_integral(int type, const Panel& I, const Panel& J) {
+    int s1, s2;
+    if (type == I2_ATTACHED_NEIGHBORS) {
+        shifts_edge(I, J, &s1, &s2);
+        return integrate_sing_edge(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                   vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    }
+    shifts_vertex(I, J, &s1, &s2);
+    return integrate_sing_vertex(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                 vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+}
+
+// Rule held in LDS: (L1, L2, L3, w) per point
+struct RuleLDS { double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS]; };
+
+__device__ __forceinline__ void load_rule(RuleLDS& s, const Rule& R) {
+    for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
+        s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
+    }
+}
+
+// Partial quadrature sum_k w_k f(r_k) of one sub-panel over points k = s, s + step, ...
+template <int TYPE>
+__device__ __forceinline__ q4 quad_part(const RuleLDS& R, int npts, int s, int step, d3 a, d3 b, d3 c, const JGeom& J,
+                                        const Sing<TYPE>& sg) {
+    q4 acc = q4zero();
+    for (int k = s; k < npts; k += step) {
+        d3 p = R.L1[k] * a;
+        p = p + R.L2[k] * b;
+        p = p + R.L3[k] * c;
+        q4 f = theta_psi_fast(p, J);
+        if (TYPE != I2_NOT_NEIGHBORS) f = f - sg.at(p);
+        acc = acc + R.w[k] * f;
+    }
+    return acc;
+}
+
+__device__ __forceinline__ q4 shfl_sum(q4 v, int G) {
+    for (int m = 1; m < G; m <<= 1) {
+        v.x += __shfl_xor(v.x, m);
+        v.y += __shfl_xor(v.y, m);
+        v.z += __shfl_xor(v.z, m);
+        v.w += __shfl_xor(v.w, m);
+    }
+    return v;
+}
+
+// Result of one task step: finished at iteration c with I_c = cur and the
+// value the reference's other buffer holds, prev (= I_{c-1}; = I_c when no
+// comparison applies or the buffer swap is not reproduced).
+struct StepOut {
+    bool finished;
+    bool pushed;
+    int c;
+    q4 cur, prev;
+};
+
+__device__ __forceinline__ void push_deep(const BatchArgs& A, int level, int type, int m, int i, int j, int64_t z, q4 prev) {
+    const int qi = level * 3 + type;
+    const unsigned long long slot = atomicAdd(&A.counters[1 + qi], 1ull);
+    if (slot >= (unsigned long long)A.qcap[type]) return;   // counted: the host grows the queue and runs again
+    QEntry e;
+    e.m = m; e.i = i; e.j = j; e.pad = 0; e.z = z; e.pad2 = 0;
+    e.prev = make_double4(prev.x, prev.y, prev.z, prev.w);
+    A.queue[A.qoff[qi] + (int64_t)slot] = e;
+}
+
+// Writes a finished task: per-pair arrays, or its contribution to the row sums
+// of both parities of K (returned in rp0/rp1 as (J, Theta)).
+template <int TYPE, int OUT>
+__device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& M, int64_t z, const StepOut& o,
+                                            const Panel& I, const Panel& J, q4* rp0, q4* rp1) {
+    q4 cur = o.cur, prev = o.prev;
+    if (TYPE != I2_NOT_NEIGHBORS) {
+        const q4 s = singular_integral(TYPE, I, J);
+        cur = cur + s;
+        prev = prev + s;
+    }
+    if (OUT == OUT_PER_PAIR) {
+        M.fin[TYPE][z] = make_double4(cur.x, cur.y, cur.z, cur.w);
+        M.prev[TYPE][z] = make_double4(prev.x, prev.y, prev.z, prev.w);
+        M.conv[TYPE][z] = (int8_t)o.c;
+    } else {
+        // K even selects I_c for even c, I_{c-1} for odd c; K odd the opposite
+        const q4 v0 = ((o.c & 1) == 0) ? cur : prev;
+        const q4 v1 = ((o.c & 1) == 1) ? cur : prev;
+        const d3 r0 = (TYPE == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(v0, J.n, I.area) : finalize_plain(v0, J.n);
+        const d3 r1 = (TYPE == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(v1, J.n, I.area) : finalize_plain(v1, J.n);
+        *rp0 = {r0.x, r0.y, r0.z, v0.w};
+        *rp1 = {r1.x, r1.y, r1.z, v1.w};
+    }
+}
+
+__device__ __forceinline__ void add_row(const MeshDesc& M, int type, int i, q4 rp0, q4 rp1) {
+    double* a0 = M.rowacc + (((size_t)type * 2 + 0) * M.nT + i) * 4;
+    double* a1 = M.rowacc + (((size_t)type * 2 + 1) * M.nT + i) * 4;
+    atomicAdd(a0 + 0, rp0.x); atomicAdd(a0 + 1, rp0.y); atomicAdd(a0 + 2, rp0.z); atomicAdd(a0 + 3, rp0.w);
+    atomicAdd(a1 + 0, rp1.x); atomicAdd(a1 + 1, rp1.y); atomicAdd(a1 + 2, rp1.z); atomicAdd(a1 + 3, rp1.w);
+}
+
+// Levels 0 and 1 of one task (lane group of LPP lanes), then the Runge check.
+template <int TYPE, int LPP>
+__device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS& R, int m, int i, int j, int64_t z, bool valid,
+                                              int s, const Panel& I, const Panel& J) {
+    Sing<TYPE> sg;
+    if (TYPE != I2_NOT_NEIGHBORS && valid) sg.init(I, J);
+    const JGeom JG = jgeom(J);
+    const int npts = A.rule.n;
+    const bool need0 = !A.fixed || A.last_level == 0;
+    const bool need1 = A.fixed ? A.last_level == 1 : A.last_level >= 1;
+    q4 v0 = q4zero(), v1 = q4zero();
+    if (valid) {
+        if (need0) v0 = I.area * quad_part<TYPE>(R, npts, s, LPP, I.a, I.b, I.c, JG, sg);
+        if (need1) {
+            const double a1 = 0.25 * I.area;
+            const d3 mBC = 0.5 * (I.b + I.c), mCA = 0.5 * (I.c + I.a), mAB = 0.5 * (I.a + I.b);
+            v1 = v1 + a1 * quad_part<TYPE>(R, npts, s, LPP, mAB, I.b, mBC, JG, sg);
+            v1 = v1 + a1 * quad_part<TYPE>(R, npts, s, LPP, mBC, I.c, mCA, JG, sg);
+            v1 = v1 + a1 * quad_part<TYPE>(R, npts, s, LPP, mCA, I.a, mAB, JG, sg);
+            v1 = v1 + a1 * quad_part<TYPE>(R, npts, s, LPP, mBC, mCA, mAB, JG, sg);
+        }
+    }
+    if (LPP > 1) {
+        v0 = shfl_sum(v0, LPP);
+        v1 = shfl_sum(v1, LPP);
+    }
+    StepOut o{false, false, 0, v0, v0};
+    if (!valid || s != 0) return o;
+    if (A.fixed) {
+        if (A.last_level <= 1) { o.finished = true; o.c = A.last_level; o.cur = o.prev = (A.last_level == 0 ? v0 : v1); }
+        else { push_deep(A, A.last_level, TYPE, m, i, j, z, v1); o.pushed = true; }
+        return o;
+    }
+    if (A.last_level == 0) { o.finished = true; o.c = 0; o.cur = o.prev = v0; return o; }
+    const double crit = runge_criterion(v1, v0, A.rule.pow2p);
+    if (!(crit > A.tol) || A.last_level == 1) { o.finished = true; o.c = 1; o.cur = v1; o.prev = A.quirk ? v0 : v1; }
+    else { push_deep(A, 2, TYPE, m, i, j, z, v1); o.pushed = true; }
+    return o;
+}
+
+// Disjoint pairs, levels 0 and 1: item = (mesh, control panel i, P consecutive j),
+// processed in rounds of blockDim/LPP pairs.
+template <int OUT, int LPP>
+__global__ __launch_bounds__(256) void k_sweep_disjoint(BatchArgs A) {
+    __shared__ RuleLDS R;
+    __shared__ long long item_sh;
+    load_rule(R, A.rule);
+    const int P = A.pairs_per_item;
+    const int per_round = blockDim.x / LPP;
+    const int p0 = threadIdx.x / LPP, s = threadIdx.x % LPP;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_sh = (long long)atomicAdd(&A.counters[0], 1ull);
+        __syncthreads();
+        const int64_t item = uniform64(item_sh);
+        if (item >= A.disjoint_items) break;
+        const int m = __builtin_amdgcn_readfirstlane(find_base(A.item_base, A.nmeshes, item));
+        const MeshDesc& M = A.meshes[m];
+        const int64_t r = item - A.item_base[m];
+        const int tiles = (M.nT + P - 1) / P;
+        const int i = __builtin_amdgcn_readfirstlane((int)(r / tiles));
+        const Panel I = M.panels[i];
+        for (int round = 0; round < P; round += per_round) {
+        const int p = round + p0;
+        const int j = (int)(r % tiles) * P + p;
+        bool valid = p < P && j < M.nT && j != i;
+        Panel J = I;
+        if (valid) {
+            J = M.panels[j];
+            valid = shared_count(I, J) == 0;
+        }
+        const int64_t z = valid ? disjoint_task(M, i, j) : 0;
+        const StepOut o = sweep_task<I2_NOT_NEIGHBORS, LPP>(A, R, m, i, j, z, valid, s, I, J);
+        q4 rp0 = q4zero(), rp1 = q4zero();
+        if (o.finished) finish_task<I2_NOT_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
+        if (OUT == OUT_ROW_SUMS) {
+            // i is uniform: reduce over the wavefront (or the whole smaller
+            // workgroup), one set of atomics per wavefront
+            const int W = blockDim.x < 64 ? (int)blockDim.x : 64;
+            rp0 = shfl_sum(rp0, W);
+            rp1 = shfl_sum(rp1, W);
+            if ((threadIdx.x & (W - 1)) == 0) add_row(M, I2_NOT_NEIGHBORS, i, rp0, rp1);
+        }
+        }
+    }
+}
+
+// Vertex- and edge-sharing tasks, levels 0 and 1: one task per lane, 256 per item.
+template <int OUT>
+__global__ __launch_bounds__(256) void k_sweep_singular(BatchArgs A) {
+    __shared__ RuleLDS R;
+    __shared__ long long item_sh;
+    load_rule(R, A.rule);
+    const int P = blockDim.x;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_sh = (long long)atomicAdd(&A.counters[1 + MAX_LEVELS * 3], 1ull);
+        __syncthreads();
+        const int64_t item = uniform64(item_sh);
+        if (item >= A.singular_items) break;
+        const int b = __builtin_amdgcn_readfirstlane(find_base(A.sing_base, 2 * A.nmeshes, item));
+        const int type = b / A.nmeshes == 0 ? I2_SIMPLE_NEIGHBORS : I2_ATTACHED_NEIGHBORS;
+        const int m = b % A.nmeshes;
+        const MeshDesc& M = A.meshes[m];
+        const int64_t n = M.npairs[type];
+        const int64_t z = (item - A.sing_base[b]) * P + threadIdx.x;
+        const bool valid = z < 2 * n;
+        int i = 0, j = 0;
+        if (valid) {
+            const int32_t* L = M.lists[type];
+            const int64_t k = z < n ? z : z - n;
+            i = L[3 * k]; j = L[3 * k + 1];
+            if (z >= n) { const int t = i; i = j; j = t; }
+        }
+        const Panel I = M.panels[i];
+        const Panel J = M.panels[j];
+        q4 rp0 = q4zero(), rp1 = q4zero();
+        if (type == I2_SIMPLE_NEIGHBORS) {
+            const StepOut o = sweep_task<I2_SIMPLE_NEIGHBORS, 1>(A, R, m, i, j, z, valid, 0, I, J);
+            if (o.finished) {
+                finish_task<I2_SIMPLE_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
+                if (OUT == OUT_ROW_SUMS) add_row(M, type, i, rp0, rp1);
+            }
+        } else {
+            const StepOut o = sweep_task<I2_ATTACHED_NEIGHBORS, 1>(A, R, m, i, j, z, valid, 0, I, J);
+            if (o.finished) {
+                finish_task<I2_ATTACHED_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
+                if (OUT == OUT_ROW_SUMS) add_row(M, type, i, rp0, rp1);
+            }
+        }
+    }
+}
+
+// Refinement level >= 2 of one queued task.
+template <int TYPE, int OUT>
+__device__ __forceinline__ void deep_task(const BatchArgs& A, const RuleLDS& R, int level, int G, int s, const QEntry& e, bool valid) {
+    const MeshDesc& M = A.meshes[e.m];
+    const Panel I = M.panels[e.i];
+    const Panel J = M.panels[e.j];
+    Sing<TYPE> sg;
+    if (TYPE != I2_NOT_NEIGHBORS && valid) sg.init(I, J);
+    const JGeom JG = jgeom(J);
+    q4 acc = q4zero();
+    if (valid) {
+        double area = I.area;
+        for (int l = 0; l < level; ++l) area *= 0.25;
+        const uint32_t nsub = 1u << (2 * level);
+        for (uint32_t c = s; c < nsub; c += G) {
+            d3 a = I.a, b = I.b, cc = I.c;
+            sub_panel(a, b, cc, level, c);
+            acc = acc + area * quad_part<TYPE>(R, A.rule.n, 0, 1, a, b, cc, JG, sg);
+        }
+    }
+    acc = shfl_sum(acc, G);
+    if (!valid || s != 0) return;
+    const q4 prev = {e.prev.x, e.prev.y, e.prev.z, e.prev.w};
+    StepOut o{false, false, level, acc, acc};
+    if (A.fixed) {
+        o.finished = true;   // fixed mode queues tasks straight at the fixed level
+    } else {
+        const double crit = runge_criterion(acc, prev, A.rule.pow2p);
+        if (!(crit > A.tol) || level == A.last_level) { o.finished = true; o.prev = A.quirk ? prev : acc; }
+        else push_deep(A, level + 1, TYPE, e.m, e.i, e.j, e.z, acc);
+    }
+    if (o.finished) {
+        q4 rp0, rp1;
+        finish_task<TYPE, OUT>(A, M, e.z, o, I, J, &rp0, &rp1);
+        if (OUT == OUT_ROW_SUMS) add_row(M, TYPE, e.i, rp0, rp1);
+    }
+}
+
+// Refinement level L >= 2 of the queued tasks of the types in [t0, t1].
+template <int OUT, int T0, int T1>
+__global__ __launch_bounds__(256) void k_deep(BatchArgs A, int level) {
+    __shared__ RuleLDS R;
+    load_rule(R, A.rule);
+    __syncthreads();
+    const int G = level >= 3 ? 64 : 16;                 // lanes per task
+    const int per_block = blockDim.x / G;
+    const int slot = threadIdx.x / G, s = threadIdx.x % G;
+    for (int type = T0; type <= T1; ++type) {
+        const int qi = level * 3 + type;
+        const unsigned long long cnt = min(A.counters[1 + qi], (unsigned long long)A.qcap[type]);
+        const QEntry* q = A.queue + A.qoff[qi];
+        for (unsigned long long base = (unsigned long long)blockIdx.x * per_block; base < cnt;
+             base += (unsigned long long)gridDim.x * per_block) {
+            const unsigned long long t = base + slot;
+            const bool valid = t < cnt;
+            QEntry e = q[valid ? t : base];
+            if (type == I2_NOT_NEIGHBORS) deep_task<I2_NOT_NEIGHBORS, OUT>(A, R, level, G, s, e, valid);
+            else if (type == I2_SIMPLE_NEIGHBORS) deep_task<I2_SIMPLE_NEIGHBORS, OUT>(A, R, level, G, s, e, valid);
+            else deep_task<I2_ATTACHED_NEIGHBORS, OUT>(A, R, level, G, s, e, valid);
+        }
+    }
+}
+
+// Which meshes have tasks at each level >= 2: OR of mesh bits over the queues,
+// reduced per wavefront before the atomics.
+__global__ __launch_bounds__(256) void k_levels(BatchArgs A) {
+    for (int L = 2; L <= A.last_level; ++L)
+        for (int type = 0; type < 3; ++type) {
+            const int qi = L * 3 + type;
+            const unsigned long long cnt = min(A.counters[1 + qi], (unsigned long long)A.qcap[type]);
+            const QEntry* q = A.queue + A.qoff[qi];
+            unsigned long long mask[MASK_WORDS];
+            for (int w = 0; w < MASK_WORDS; ++w) mask[w] = 0ull;
+            for (unsigned long long t = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; t < cnt;
+                 t += (unsigned long long)gridDim.x * blockDim.x) {
+                const int m = q[t].m;
+                mask[m >> 6] |= 1ull << (m & 63);
+            }
+            for (int w = 0; w < MASK_WORDS; ++w) {
+                unsigned long long v = mask[w];
+                for (int o = 1; o < 64; o <<= 1) v |= __shfl_xor(v, o);
+                if ((threadIdx.x & 63) == 0 && v) atomicOr(&A.levelmask[qi * MASK_WORDS + w], v);
+            }
+        }
+}
+
+// K per (mesh, type): 0 when maxLevel = 0 (automatic) or no task; else the
+// deepest level any of its tasks reached (at least 1).  Fixed mode: the level.
+__global__ void k_compute_K(BatchArgs A) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 3 * A.nmeshes) return;
+    const int m = g / 3, type = g % 3;
+    const bool any = A.meshes[m].npairs[type] > 0;
+    int K = 0;
+    if (A.fixed) K = A.last_level;
+    else if (any && A.last_level >= 1) {
+        K = 1;
+        for (int L = 2; L <= A.last_level; ++L)
+            if ((A.levelmask[(L * 3 + type) * MASK_WORDS + (m >> 6)] >> (m & 63)) & 1ull) K = L;
+    }
+    A.K[g] = K;
+}
+
+// Parity selection and J for per-pair outputs, grid-stride over all tasks.
+__global__ __launch_bounds__(256) void k_finalize_pairs(BatchArgs A) {
+    const int64_t total = A.pair_base[3 * A.nmeshes];
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int b = find_base(A.pair_base, 3 * A.nmeshes, g);
+        const int m = b / 3, type = b % 3;
+        const MeshDesc& M = A.meshes[m];
+        const int64_t z = g - A.pair_base[b];
+        const int64_t n = M.npairs[type];
+        const int64_t k = z < n ? z : z - n;
+        int i, j;
+        if (type == I2_NOT_NEIGHBORS) {
+            const int ii = find_base(M.row_start, M.nT, k);
+            int64_t jj = ii + 1 + (k - M.row_start[ii]);
+            for (int64_t q = M.nbr_start[ii]; q < M.nbr_start[ii + 1]; ++q) {
+                if (M.nbr[q] <= jj) ++jj; else break;
+            }
+            i = ii; j = (int)jj;
+        } else {
+            i = M.lists[type][3 * k]; j = M.lists[type][3 * k + 1];
+        }
+        if (z >= n) { const int t = i; i = j; j = t; }
+        const int K = A.K[3 * m + type];
+        const int c = M.conv[type][z];
+        const double4 v4 = (A.quirk && c >= 1 && ((K - c) & 1)) ? M.prev[type][z] : M.fin[type][z];
+        const q4 v = {v4.x, v4.y, v4.z, v4.w};
+        const d3 nJ = M.panels[j].n;
+        const double areaI = M.panels[i].area;
+        const d3 r = (type == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(v, nJ, areaI) : finalize_plain(v, nJ);
+        M.fin[type][z] = v4;
+        double* o = M.res[type] + 3 * z;
+        o[0] = r.x; o[1] = r.y; o[2] = r.z;
+    }
+}
+
+// Row sums for K's parity, summed over the three types: (sum_j J, sum_j Theta).
+__global__ void k_finalize_rows(BatchArgs A, int m) {
+    const MeshDesc& M = A.meshes[m];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.nT) return;
+    double r[4] = {0, 0, 0, 0};
+    for (int type = 0; type < 3; ++type) {
+        const int par = A.K[3 * m + type] & 1;
+        const double* acc = M.rowacc + (((size_t)type * 2 + par) * M.nT + i) * 4;
+        for (int c = 0; c < 4; ++c) r[c] += acc[c];
+    }
+    for (int c = 0; c < 4; ++c) M.rowout[4 * (size_t)i + c] = r[c];
+}
+
+// Zero-fill kernels (graph-safe replacement of hipMemsetAsync).
+__global__ void k_zero_u64(unsigned long long* p, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) p[q] = 0ull;
+}
+__global__ void k_zero_f64(double* p, int64_t n) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) p[q] = 0.0;
+}
+
+// ------------------------------------------------------------ host engine
+int lanes_for_points_per_lane(int ppl) {
+    switch (ppl) {
+        case 1: return 16;   // 13 points over 16 lanes
+        case 2: return 8;
+        case 4: return 4;
+        default: return 1;   // the whole rule in one lane
+    }
+}
+
+// workgroup of the disjoint sweep: P pairs of LPP lanes, at most 256 lanes
+static int sweep_block(int P, int lpp) { return std::min(256, P * lpp); }
+
+template <int OUT, int LPP>
+static int sweep_grid_for(int P, int* grid) {
+    int dev = 0, cus = 0, per = 0;
+    I2_HIP(hipGetDevice(&dev));
+    I2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    I2_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sweep_disjoint<OUT, LPP>, sweep_block(P, LPP), 0));
+    *grid = std::max(1, cus * std::max(1, per));
+    return I2_OK;
+}
+
+static int launch_sweep(const Engine& E, hipStream_t st) {
+    const int lpp = lanes_for_points_per_lane(E.ppl);
+    const dim3 blk(sweep_block(E.pairs_per_wg, lpp)), grd(E.sweep_grid);
+#define I2_SWEEP(O, L) hipLaunchKernelGGL((k_sweep_disjoint<O, L>), grd, blk, 0, st, E.args)
+    if (E.out == OUT_PER_PAIR) {
+        if (lpp == 1) I2_SWEEP(OUT_PER_PAIR, 1); else if (lpp == 4) I2_SWEEP(OUT_PER_PAIR, 4);
+        else if (lpp == 8) I2_SWEEP(OUT_PER_PAIR, 8); else I2_SWEEP(OUT_PER_PAIR, 16);
+    } else {
+        if (lpp == 1) I2_SWEEP(OUT_ROW_SUMS, 1); else if (lpp == 4) I2_SWEEP(OUT_ROW_SUMS, 4);
+        else if (lpp == 8) I2_SWEEP(OUT_ROW_SUMS, 8); else I2_SWEEP(OUT_ROW_SUMS, 16);
+    }
+#undef I2_SWEEP
+    I2_TRY(check_launch("k_sweep_disjoint"));
+    if (E.args.singular_items > 0) {
+        if (E.out == OUT_PER_PAIR) hipLaunchKernelGGL((k_sweep_singular<OUT_PER_PAIR>), dim3(E.deep_grid), dim3(256), 0, st, E.args);
+        else hipLaunchKernelGGL((k_sweep_singular<OUT_ROW_SUMS>), dim3(E.deep_grid), dim3(256), 0, st, E.args);
+        I2_TRY(check_launch("k_sweep_singular"));
+    }
+    return I2_OK;
+}
+
+static int compute_grids(Engine& E) {
+    const int lpp = lanes_for_points_per_lane(E.ppl);
+    if (E.pairs_per_wg < 1 || E.pairs_per_wg > 4096 || (E.pairs_per_wg * lpp > 256 && (E.pairs_per_wg * lpp) % 256 != 0)) {
+        set_error("pairs per workgroup %d not usable with %d lanes per pair", E.pairs_per_wg, lpp);
+        return I2_ERR_INVALID;
+    }
+#define I2_GRID(O, L) sweep_grid_for<O, L>(E.pairs_per_wg, &E.sweep_grid)
+    if (E.out == OUT_PER_PAIR) {
+        if (lpp == 1) I2_TRY(I2_GRID(OUT_PER_PAIR, 1)); else if (lpp == 4) I2_TRY(I2_GRID(OUT_PER_PAIR, 4));
+        else if (lpp == 8) I2_TRY(I2_GRID(OUT_PER_PAIR, 8)); else I2_TRY(I2_GRID(OUT_PER_PAIR, 16));
+    } else {
+        if (lpp == 1) I2_TRY(I2_GRID(OUT_ROW_SUMS, 1)); else if (lpp == 4) I2_TRY(I2_GRID(OUT_ROW_SUMS, 4));
+        else if (lpp == 8) I2_TRY(I2_GRID(OUT_ROW_SUMS, 8)); else I2_TRY(I2_GRID(OUT_ROW_SUMS, 16));
+    }
+#undef I2_GRID
+    int dev = 0, cus = 0;
+    I2_HIP(hipGetDevice(&dev));
+    I2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    E.deep_grid = cus * 8;
+    E.fin_grid = cus * 16;
+    return I2_OK;
+}
+
+int batch_enqueue(Engine& E, hipStream_t st) {
+    const BatchArgs& A = E.args;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, st, E.d_counters.p, (int64_t)E.d_counters.n);
+    I2_TRY(check_launch("k_zero_u64"));
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, st, E.d_levelmask.p, (int64_t)E.d_levelmask.n);
+    I2_TRY(check_launch("k_zero_u64"));
+    I2_TRY(launch_sweep(E, st));
+    for (int L = 2; L <= A.last_level; ++L) {
+        if (A.fixed && L != A.last_level) continue;   // fixed mode queues tasks at its level only
+        if (E.out == OUT_PER_PAIR) {
+            hipLaunchKernelGGL((k_deep<OUT_PER_PAIR, 2, 2>), dim3(E.deep_grid), dim3(256), 0, st, A, L);
+            I2_TRY(check_launch("k_deep"));
+            hipLaunchKernelGGL((k_deep<OUT_PER_PAIR, 0, 1>), dim3(E.deep_grid), dim3(256), 0, st, A, L);
+        } else {
+            hipLaunchKernelGGL((k_deep<OUT_ROW_SUMS, 2, 2>), dim3(E.deep_grid), dim3(256), 0, st, A, L);
+            I2_TRY(check_launch("k_deep"));
+            hipLaunchKernelGGL((k_deep<OUT_ROW_SUMS, 0, 1>), dim3(E.deep_grid), dim3(256), 0, st, A, L);
+        }
+        I2_TRY(check_launch("k_deep"));
+    }
+    if (A.last_level >= 2) {
+        hipLaunchKernelGGL(k_levels, dim3(E.deep_grid), dim3(256), 0, st, A);
+        I2_TRY(check_launch("k_levels"));
+    }
+    hipLaunchKernelGGL(k_compute_K, dim3((3 * A.nmeshes + 255) / 256), dim3(256), 0, st, A);
+    I2_TRY(check_launch("k_compute_K"));
+    if (E.out == OUT_PER_PAIR) {
+        hipLaunchKernelGGL(k_finalize_pairs, dim3(E.fin_grid), dim3(256), 0, st, A);
+        I2_TRY(check_launch("k_finalize_pairs"));
+    } else {
+        for (int m = 0; m < A.nmeshes; ++m) {
+            const int nT = (int)E.nT[m];
+            hipLaunchKernelGGL(k_finalize_rows, dim3((nT + 255) / 256), dim3(256), 0, st, A, m);
+            I2_TRY(check_launch("k_finalize_rows"));
+        }
+    }
+    return I2_OK;
+}
+
+int engine_setup(Engine& E, const i2_integrator& it) {
+    const int M = (int)E.h_meshes.size();
+    if (M < 1 || M > MAX_BATCH_MESHES) { set_error("a batch holds 1..%d meshes", MAX_BATCH_MESHES); return I2_ERR_RANGE; }
+    BatchArgs& A = E.args;
+    A.nmeshes = M;
+    A.fixed = it.fixed;
+    A.last_level = it.fixed ? it.fixed_level : it.max_level;
+    A.quirk = it.quirk;
+    A.tol = it.tol;
+    A.rule = it.rule;
+    A.pairs_per_item = E.pairs_per_wg;
+    if (A.last_level >= MAX_LEVELS) { set_error("refinement level above %d", MAX_LEVELS - 1); return I2_ERR_RANGE; }
+    I2_TRY(compute_grids(E));
+    const int P = E.pairs_per_wg;
+    // sweep items (disjoint per mesh; singular per (type, mesh), 256 tasks each) and per-pair ranges
+    E.h_item_base.assign(M + 1, 0);
+    E.h_sing_base.assign(2 * M + 1, 0);
+    E.h_pair_base.assign(3 * M + 1, 0);
+    E.nT.resize(M);
+    int64_t acc = 0;
+    for (int m = 0; m < M; ++m) {
+        const MeshDesc& D = E.h_meshes[m];
+        E.h_item_base[m] = acc;
+        acc += (int64_t)D.nT * ((D.nT + P - 1) / P);
+    }
+    E.h_item_base[M] = acc;
+    A.disjoint_items = acc;
+    acc = 0;
+    for (int kind = 0; kind < 2; ++kind)
+        for (int m = 0; m < M; ++m) {
+            E.h_sing_base[kind * M + m] = acc;
+            acc += (2 * E.h_meshes[m].npairs[kind] + 255) / 256;
+        }
+    E.h_sing_base[2 * M] = acc;
+    A.singular_items = acc;
+    int64_t pb = 0;
+    E.ordered[0] = E.ordered[1] = E.ordered[2] = 0;
+    for (int m = 0; m < M; ++m) {
+        E.nT[m] = E.h_meshes[m].nT;
+        for (int t = 0; t < 3; ++t) {
+            E.h_pair_base[3 * m + t] = pb;
+            pb += 2 * E.h_meshes[m].npairs[t];
+            E.ordered[t] += 2 * E.h_meshes[m].npairs[t];
+        }
+    }
+    E.h_pair_base[3 * M] = pb;
+    E.total_tasks = pb;
+    // queue segments: levels 2..last, capacity per type (grown after an overflow)
+    for (int t = 0; t < 3; ++t)
+        if (A.qcap[t] <= 0) A.qcap[t] = std::max<int64_t>(1, std::min<int64_t>(E.ordered[t], std::max<int64_t>(65536, E.ordered[t] / 8)));
+    E.h_qoff.assign(MAX_LEVELS * 3, 0);
+    int64_t qtot = 0;
+    for (int L = 2; L <= A.last_level; ++L)
+        for (int t = 0; t < 3; ++t) { E.h_qoff[L * 3 + t] = qtot; qtot += A.qcap[t]; }
+    I2_TRY(E.d_queue.ensure(std::max<int64_t>(1, qtot)));
+    I2_TRY(E.d_qoff.ensure((int64_t)E.h_qoff.size()));
+    I2_TRY(E.d_meshes.ensure(M));
+    I2_TRY(E.d_item_base.ensure(M + 1));
+    I2_TRY(E.d_sing_base.ensure(2 * M + 1));
+    I2_TRY(E.d_pair_base.ensure(3 * M + 1));
+    I2_TRY(E.d_counters.ensure(2 + MAX_LEVELS * 3));
+    I2_TRY(E.d_levelmask.ensure(MAX_LEVELS * 3 * MASK_WORDS));
+    I2_TRY(E.d_K.ensure(3 * M));
+    I2_HIP(hipMemcpy(E.d_qoff.p, E.h_qoff.data(), sizeof(int64_t) * E.h_qoff.size(), hipMemcpyHostToDevice));
+    I2_HIP(hipMemcpy(E.d_meshes.p, E.h_meshes.data(), sizeof(MeshDesc) * M, hipMemcpyHostToDevice));
+    I2_HIP(hipMemcpy(E.d_item_base.p, E.h_item_base.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice));
+    I2_HIP(hipMemcpy(E.d_sing_base.p, E.h_sing_base.data(), sizeof(int64_t) * (2 * M + 1), hipMemcpyHostToDevice));
+    I2_HIP(hipMemcpy(E.d_pair_base.p, E.h_pair_base.data(), sizeof(int64_t) * (3 * M + 1), hipMemcpyHostToDevice));
+    A.meshes = E.d_meshes.p;
+    A.item_base = E.d_item_base.p;
+    A.sing_base = E.d_sing_base.p;
+    A.pair_base = E.d_pair_base.p;
+    A.counters = E.d_counters.p;
+    A.queue = E.d_queue.p;
+    A.qoff = E.d_qoff.p;
+    A.levelmask = E.d_levelmask.p;
+    A.K = E.d_K.p;
+    return I2_OK;
+}
+
+int engine_check_queues(Engine& E, hipStream_t st, bool* overflow, std::vector<unsigned long long>* counts) {
+    std::vector<unsigned long long> c(2 + MAX_LEVELS * 3);
+    I2_HIP(hipMemcpyAsync(c.data(), E.d_counters.p, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost, st));
+    I2_HIP(hipStreamSynchronize(st));
+    *overflow = false;
+    int64_t need[3] = {0, 0, 0};
+    for (int L = 2; L <= E.args.last_level; ++L)
+        for (int t = 0; t < 3; ++t) need[t] = std::max<int64_t>(need[t], (int64_t)c[1 + L * 3 + t]);
+    for (int t = 0; t < 3; ++t)
+        if (need[t] > E.args.qcap[t]) { *overflow = true; E.args.qcap[t] = need[t] + need[t] / 4 + 1; }
+    if (counts) *counts = c;
+    return I2_OK;
+}
+
+}  // namespace i2
+
+// ------------------------------------------------------------ i2_batch objects
+struct i2_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<const i2_mesh*> meshes;
+    i2_integrator cfg{};
+    bool have_cfg = false;
+    int out = i2::OUT_PER_PAIR;
+    int P = 64, ppl = 13;
+    i2::Engine E;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    // owned outputs per mesh
+    struct Out {
+        i2::DBuf<double4> fin[3], prev[3];
+        i2::DBuf<int8_t> conv[3];
+        i2::DBuf<double> res[3];
+        i2::DBuf<double> rowacc, rowout;
+    };
+    std::vector<Out> outs;
+    std::vector<int> K;
+    std::vector<unsigned long long> counts;
+    bool ran = false;
+};
+
+namespace i2 {
+
+static MeshDesc desc_of(const i2_mesh* m) {
+    MeshDesc D{};
+    D.panels = m->d_panels.p;
+    D.row_start = m->d_row_start.p;
+    D.nbr_start = m->d_nbr_start.p;
+    D.nbr = m->d_nbr.p;
+    D.lists[0] = m->d_simple.p;
+    D.lists[1] = m->d_attached.p;
+    for (int t = 0; t < 3; ++t) D.npairs[t] = m->pairs[t];
+    D.nT = (int)m->nT;
+    return D;
+}
+
+static void drop_graph(i2_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static int batch_prepare(i2_batch* b) {
+    Engine& E = b->E;
+    E.out = b->out;
+    E.pairs_per_wg = b->P;
+    E.ppl = b->ppl;
+    const int M = (int)b->meshes.size();
+    b->outs.resize(M);
+    E.h_meshes.resize(M);
+    for (int m = 0; m < M; ++m) {
+        const i2_mesh* mesh = b->meshes[m];
+        if (!mesh->prepared) { set_error("mesh %d is not prepared", m); return I2_ERR_STATE; }
+        if (mesh->device != b->device) { set_error("mesh %d lives on device %d, batch on %d", m, mesh->device, b->device); return I2_ERR_INVALID; }
+        MeshDesc D = desc_of(mesh);
+        auto& o = b->outs[m];
+        if (b->out == OUT_PER_PAIR) {
+            for (int t = 0; t < 3; ++t) {
+                const int64_t n = 2 * mesh->pairs[t];
+                I2_TRY(o.fin[t].ensure(n)); I2_TRY(o.prev[t].ensure(n)); I2_TRY(o.conv[t].ensure(n)); I2_TRY(o.res[t].ensure(3 * n));
+                D.fin[t] = o.fin[t].p; D.prev[t] = o.prev[t].p; D.conv[t] = o.conv[t].p; D.res[t] = o.res[t].p;
+            }
+        } else {
+            I2_TRY(o.rowacc.ensure(24 * mesh->nT));
+            I2_TRY(o.rowout.ensure(4 * mesh->nT));
+            D.rowacc = o.rowacc.p;
+            D.rowout = o.rowout.p;
+        }
+        E.h_meshes[m] = D;
+    }
+    return engine_setup(E, b->cfg);
+}
+
+static int batch_enqueue_all(i2_batch* b) {
+    Engine& E = b->E;
+    if (E.out == OUT_ROW_SUMS)
+        for (auto& o : b->outs) {
+            hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, b->stream, o.rowacc.p, (int64_t)o.rowacc.n);
+            I2_TRY(check_launch("k_zero_f64"));
+        }
+    return batch_enqueue(E, b->stream);
+}
+
+static int batch_capture(i2_batch* b) {
+    drop_graph(b);
+    I2_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int st = batch_enqueue_all(b);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (st != I2_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (ec != hipSuccess) { set_error("graph capture failed: %s", hipGetErrorString(ec)); return I2_ERR_HIP; }
+    b->graph = g;
+    I2_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return I2_OK;
+}
+
+static int batch_run(i2_batch* b, double* ms) {
+    DeviceGuard g(b->device);
+    if (!g.ok) return I2_ERR_HIP;
+    if (b->meshes.empty()) { set_error("batch has no meshes"); return I2_ERR_STATE; }
+    if (!b->have_cfg) { set_error("batch not configured (i2_batch_configure)"); return I2_ERR_STATE; }
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        if (b->dirty) {
+            I2_TRY(batch_prepare(b));
+            I2_TRY(batch_capture(b));
+            b->dirty = false;
+        }
+        I2_HIP(hipEventRecord(b->e0, b->stream));
+        I2_HIP(hipGraphLaunch(b->exec, b->stream));
+        I2_HIP(hipEventRecord(b->e1, b->stream));
+        bool overflow = false;
+        I2_TRY(engine_check_queues(b->E, b->stream, &overflow, &b->counts));
+        if (!overflow) {
+            float f = 0.f;
+            I2_HIP(hipEventElapsedTime(&f, b->e0, b->e1));
+            if (ms) *ms = f;
+            b->K.resize(3 * b->meshes.size());
+            I2_HIP(hipMemcpy(b->K.data(), b->E.d_K.p, sizeof(int) * b->K.size(), hipMemcpyDeviceToHost));
+            b->ran = true;
+            return I2_OK;
+        }
+        b->dirty = true;   // queues were grown: rebuild and run again
+    }
+    set_error("refinement queues kept overflowing");
+    return I2_ERR_RANGE;
+}
+
+// The evaluator's fast path: one mesh, per-pair outputs into its type buffers.
+int engine_run_evaluator(i2_evaluator* ev) {
+    const i2_mesh* m = ev->mesh;
+    i2_batch* b = ev->fast;
+    if (!b) {
+        b = new i2_batch;
+        b->device = m->device;
+        b->stream = ev->stream;
+        if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+            (void)hipGetLastError();
+            delete b;
+            set_error("event creation failed");
+            return I2_ERR_HIP;
+        }
+        ev->fast = b;
+    }
+    if (b->meshes.size() != 1 || b->meshes[0] != m) { b->meshes.assign(1, m); b->dirty = true; }
+    const i2_integrator& it = *ev->integ;
+    if (!b->have_cfg || memcmp(&b->cfg.rule, &it.rule, sizeof(Rule)) != 0 || b->cfg.fixed != it.fixed ||
+        b->cfg.fixed_level != it.fixed_level || b->cfg.max_level != it.max_level || b->cfg.tol != it.tol || b->cfg.quirk != it.quirk) {
+        b->cfg = it;
+        b->have_cfg = true;
+        b->dirty = true;
+    }
+    if (b->P != ev->fast_P || b->ppl != ev->fast_ppl) { b->P = ev->fast_P; b->ppl = ev->fast_ppl; b->dirty = true; }
+    b->out = OUT_PER_PAIR;
+    double ms = 0.0;
+    I2_TRY(batch_run(b, &ms));
+    for (int t = 0; t < 3; ++t) {
+        TypeState& ts = ev->type[t];
+        ts.n = 2 * m->pairs[t];
+        ts.implicit = (t == I2_NOT_NEIGHBORS);
+        ts.K = b->K[t];
+        ts.final_buf = 0;
+        ts.ms = ms;
+        ts.refined_tasks = 0;
+    }
+    return I2_OK;
+}
+
+void engine_evaluator_outputs(const i2_evaluator* ev, int type, const double4** ints, const double** res, const int8_t** conv) {
+    const i2_batch* b = ev->fast;
+    *ints = b->outs[0].fin[type].p;
+    *res = b->outs[0].res[type].p;
+    *conv = b->outs[0].conv[type].p;
+}
+
+void engine_release_evaluator(i2_evaluator* ev) {
+    i2_batch* b = ev->fast;
+    if (!b) return;
+    drop_graph(b);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    b->stream = nullptr;   // owned by the evaluator
+    delete b;
+    ev->fast = nullptr;
+}
+
+}  // namespace i2
+
+using namespace i2;
+
+extern "C" {
+
+int i2_batch_create(i2_batch** out, int device) {
+    if (!out) { set_error("null pointer"); return I2_ERR_INVALID; }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) { (void)hipGetLastError(); set_error("no HIP device available"); return I2_ERR_HIP; }
+    if (device < 0 || device >= count) { set_error("device %d out of range", device); return I2_ERR_INVALID; }
+    DeviceGuard g(device);
+    if (!g.ok) return I2_ERR_HIP;
+    i2_batch* b = new i2_batch;
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+        hipEventCreate(&b->e1) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("stream/event creation failed");
+        i2_batch_destroy(b);
+        return I2_ERR_HIP;
+    }
+    *out = b;
+    return I2_OK;
+}
+
+void i2_batch_destroy(i2_batch* b) {
+    if (!b) return;
+    {
+        DeviceGuard g(b->device);
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        drop_graph(b);
+        b->outs.clear();
+        if (b->e0) (void)hipEventDestroy(b->e0);
+        if (b->e1) (void)hipEventDestroy(b->e1);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+}
+
+int i2_batch_add_mesh(i2_batch* b, const i2_mesh* m, int* index) {
+    if (!b || !m) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!m->prepared) { set_error("mesh not prepared"); return I2_ERR_STATE; }
+    if ((int)b->meshes.size() >= MAX_BATCH_MESHES) { set_error("batch is full (%d meshes)", MAX_BATCH_MESHES); return I2_ERR_RANGE; }
+    b->meshes.push_back(m);
+    if (index) *index = (int)b->meshes.size() - 1;
+    b->dirty = true;
+    b->ran = false;
+    return I2_OK;
+}
+
+int i2_batch_clear(i2_batch* b) {
+    if (!b) { set_error("null batch"); return I2_ERR_INVALID; }
+    b->meshes.clear();
+    b->dirty = true;
+    b->ran = false;
+    return I2_OK;
+}
+
+int i2_batch_configure(i2_batch* b, const i2_integrator* it, int outputMode) {
+    if (!b || !it) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (outputMode != OUT_PER_PAIR && outputMode != OUT_ROW_SUMS) { set_error("bad output mode %d", outputMode); return I2_ERR_INVALID; }
+    b->cfg = *it;
+    b->have_cfg = true;
+    b->out = outputMode;
+    b->dirty = true;
+    b->ran = false;
+    return I2_OK;
+}
+
+int i2_batch_set_tuning(i2_batch* b, int pairsPerWorkgroup, int pointsPerLane) {
+    if (!b) { set_error("null batch"); return I2_ERR_INVALID; }
+    if (pairsPerWorkgroup < 1 || pairsPerWorkgroup > 4096 || (pairsPerWorkgroup & (pairsPerWorkgroup - 1)) != 0) {
+        set_error("pairs per workgroup must be a power of two in [1, 4096]");
+        return I2_ERR_INVALID;
+    }
+    if (pointsPerLane != 1 && pointsPerLane != 2 && pointsPerLane != 4 && pointsPerLane != 13) { set_error("points per lane must be 1, 2, 4 or 13"); return I2_ERR_INVALID; }
+    b->P = pairsPerWorkgroup;
+    b->ppl = pointsPerLane;
+    b->dirty = true;
+    return I2_OK;
+}
+
+int i2_batch_run(i2_batch* b, double* ms) {
+    if (!b) { set_error("null batch"); return I2_ERR_INVALID; }
+    return batch_run(b, ms);
+}
+
+int i2_batch_get_pair_results(const i2_batch* b, int mesh, int type, double* integrals4, double* results3, int8_t* conv) {
+    if (!b || mesh < 0 || mesh >= (int)b->meshes.size() || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!b->ran || b->out != OUT_PER_PAIR) { set_error("no per-pair results"); return I2_ERR_STATE; }
+    DeviceGuard g(b->device);
+    if (!g.ok) return I2_ERR_HIP;
+    const auto& o = b->outs[mesh];
+    const int64_t n = 2 * b->meshes[mesh]->pairs[type];
+    if (n == 0) return I2_OK;
+    if (integrals4) I2_HIP(hipMemcpy(integrals4, o.fin[type].p, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    if (results3) I2_HIP(hipMemcpy(results3, o.res[type].p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    if (conv) I2_HIP(hipMemcpy(conv, o.conv[type].p, (size_t)n, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+int i2_batch_get_row_sums(const i2_batch* b, int mesh, double* rows4) {
+    if (!b || !rows4 || mesh < 0 || mesh >= (int)b->meshes.size()) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!b->ran || b->out != OUT_ROW_SUMS) { set_error("no row sums"); return I2_ERR_STATE; }
+    DeviceGuard g(b->device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_HIP(hipMemcpy(rows4, b->outs[mesh].rowout.p, sizeof(double) * 4 * b->meshes[mesh]->nT, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+int i2_batch_refinement_iterations(const i2_batch* b, int mesh, int type, int* K) {
+    if (!b || !K || mesh < 0 || mesh >= (int)b->meshes.size() || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!b->ran) { set_error("batch has not run"); return I2_ERR_STATE; }
+    *K = b->K[3 * mesh + type];
+    return I2_OK;
+}
+
+int i2_batch_level_counts(const i2_batch* b, int64_t* counts) {
+    if (!b || !counts) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!b->ran) { set_error("batch has not run"); return I2_ERR_STATE; }
+    for (int L = 0; L < MAX_LEVELS; ++L)
+        for (int t = 0; t < 3; ++t) counts[3 * L + t] = (L >= 2) ? (int64_t)b->counts[1 + 3 * L + t] : 0;
+    for (int t = 0; t < 3; ++t) {
+        int64_t n = 0;
+        for (const i2_mesh* m : b->meshes) n += 2 * m->pairs[t];
+        counts[t] = n;
+        counts[3 + t] = (b->cfg.fixed && b->cfg.fixed_level == 0) || (!b->cfg.fixed && b->cfg.max_level == 0) ? 0 : n;
+    }
+    return I2_OK;
+}
+
+int i2_batch_task_count(const i2_batch* b, int64_t* n) {
+    if (!b || !n) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    int64_t s = 0;
+    for (const i2_mesh* m : b->meshes) s += 2 * (m->pairs[0] + m->pairs[1] + m->pairs[2]);
+    *n = s;
+    return I2_OK;
+}
+
+}  // extern "C"

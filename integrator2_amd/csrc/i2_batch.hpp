@@ -1,0 +1,108 @@
+// i2_batch.hpp -- device descriptors and host engine of the batched path.
+#pragma once
+#include "i2_internal.hpp"
+
+namespace i2 {
+
+constexpr int OUT_PER_PAIR = 0;   // per-pair integrals and J in reference task order
+constexpr int OUT_ROW_SUMS = 1;   // per control panel: sum over j of J and Theta
+constexpr int MAX_LEVELS = 13;    // levels 0..12
+
+// One mesh of a batch as the kernels see it.
+struct MeshDesc {
+    const i2d::Panel* panels;
+    const int64_t* row_start;     // disjoint pair rows (lexicographic list)
+    const int64_t* nbr_start;     // per cell: sorted neighbours j > i (CSR)
+    const int32_t* nbr;
+    const int32_t* lists[2];      // (i, j, k) lists of vertex- and edge-sharing pairs, i < j
+    int64_t npairs[3];            // unordered pair counts per type
+    int nT;
+    int pad;
+    // per-pair outputs (per type, 2*npairs entries, reference task order)
+    double4* fin[3];
+    double4* prev[3];
+    int8_t* conv[3];
+    double* res[3];
+    // row-sum outputs: rowacc[type][parity][nT][4], rowout[nT][4]
+    double* rowacc;
+    double* rowout;
+};
+
+// A task waiting for refinement level >= 2.
+struct alignas(32) QEntry {
+    int m, i, j, pad;
+    int64_t z;
+    int64_t pad2;
+    double4 prev;                 // value of the previous level
+};
+static_assert(sizeof(QEntry) == 64, "queue entries are 64 bytes");
+
+constexpr int MAX_BATCH_MESHES = 256;
+constexpr int MASK_WORDS = MAX_BATCH_MESHES / 64;
+
+struct BatchArgs {
+    const MeshDesc* meshes;
+    int nmeshes;
+    int fixed;                    // 1: fixed refinement level = last_level
+    int last_level;               // maxLevel (automatic) or the fixed level
+    int quirk;
+    int pairs_per_item;           // disjoint pairs (consecutive j) per sweep item
+    double tol;
+    i2d::Rule rule;
+    // sweep items: disjoint item_base[m] (nmeshes + 1), singular sing_base[kind * nmeshes + m]
+    // with kind 0 vertex-sharing, 1 edge-sharing (2 * nmeshes + 1)
+    const int64_t* item_base;
+    const int64_t* sing_base;
+    int64_t disjoint_items;
+    int64_t singular_items;
+    // per-pair finalize ranges: pair_base[3 * m + type]
+    const int64_t* pair_base;
+    // counters: [0] disjoint item counter, [1 + level*3 + type] queue counts,
+    // [1 + MAX_LEVELS*3] singular item counter
+    unsigned long long* counters;
+    QEntry* queue;                // segment (level, type) at qoff[level*3 + type], capacity qcap[type]
+    const int64_t* qoff;
+    int64_t qcap[3];
+    // levelmask[(level*3 + type) * MASK_WORDS + w]: meshes with a task at that level
+    unsigned long long* levelmask;
+    int* K;                       // K[3*m + type]: refinement iterations run
+};
+
+// Host engine state shared by i2_batch and the evaluator's fast path.
+struct Engine {
+    int device = 0;
+    int out = OUT_PER_PAIR;
+    int pairs_per_wg = 64;
+    int ppl = 13;                 // quadrature points per lane (13 = whole rule per lane)
+    int sweep_grid = 0, deep_grid = 0, fin_grid = 0;
+    BatchArgs args{};
+    std::vector<MeshDesc> h_meshes;
+    std::vector<int64_t> h_item_base, h_sing_base, h_pair_base, h_qoff;
+    std::vector<int64_t> nT;
+    int64_t ordered[3] = {0, 0, 0};   // ordered tasks per type over the batch
+    DBuf<MeshDesc> d_meshes;
+    DBuf<int64_t> d_item_base, d_sing_base, d_pair_base, d_qoff;
+    DBuf<unsigned long long> d_counters;
+    DBuf<QEntry> d_queue;
+    DBuf<unsigned long long> d_levelmask;
+    DBuf<int> d_K;
+    int64_t total_tasks = 0;
+};
+
+int lanes_for_points_per_lane(int ppl);
+// Enqueues one run (kernels only, so the sequence can be captured in a graph).
+int batch_enqueue(Engine& E, hipStream_t stream);
+// Builds device descriptors and work queues for the meshes in E.h_meshes
+// (outputs already set) with the rule and error control of `it`.
+int engine_setup(Engine& E, const i2_integrator& it);
+// After a run: queue counts per (level, type); returns I2_ERR_RANGE-free
+// status and sets *overflow when a queue segment was too small (the queues are
+// then grown and the caller must run again).
+int engine_check_queues(Engine& E, hipStream_t stream, bool* overflow, std::vector<unsigned long long>* counts);
+// Runs an evaluator's mesh through the batched kernels (per-pair outputs).
+int engine_run_evaluator(i2_evaluator* ev);
+void engine_release_evaluator(i2_evaluator* ev);
+// Device pointers of the per-pair outputs of the evaluator's fast path.
+void engine_evaluator_outputs(const i2_evaluator* ev, int type, const double4** ints, const double** res, const int8_t** conv);
+
+}  // namespace i2

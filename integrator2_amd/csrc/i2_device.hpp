@@ -9,6 +9,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <math.h>
+
+// Device math is also compiled for the host so that the C ABI can expose
+// host-side evaluations of the same formulas for checking.
+#define I2_HD __host__ __device__
+
+#include "i2_math.hpp"
 
 namespace i2d {
 
@@ -37,22 +44,22 @@ struct Rule {
 
 // ------------------------------------------------------------ vectors
 struct d3 { double x, y, z; };
-__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ d3 operator*(double s, d3 a) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-__device__ __forceinline__ double vlen(d3 a) { return sqrt(dot(a, a)); }
-__device__ __forceinline__ d3 unit(d3 a) { const double inv = 1.0 / vlen(a); return {a.x * inv, a.y * inv, a.z * inv}; }
-__device__ __forceinline__ d3 divs(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+I2_HD __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+I2_HD __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+I2_HD __forceinline__ d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
+I2_HD __forceinline__ d3 operator*(double s, d3 a) { return {a.x * s, a.y * s, a.z * s}; }
+I2_HD __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+I2_HD __forceinline__ d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+I2_HD __forceinline__ double vlen(d3 a) { return sqrt(dot(a, a)); }
+I2_HD __forceinline__ d3 unit(d3 a) { const double inv = 1.0 / vlen(a); return {a.x * inv, a.y * inv, a.z * inv}; }
+I2_HD __forceinline__ d3 divs(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
 
 // (Psi, Theta) accumulator
 struct q4 { double x, y, z, w; };
-__device__ __forceinline__ q4 q4zero() { return {0.0, 0.0, 0.0, 0.0}; }
-__device__ __forceinline__ q4 operator+(q4 a, q4 b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
-__device__ __forceinline__ q4 operator-(q4 a, q4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
-__device__ __forceinline__ q4 operator*(double s, q4 a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+I2_HD __forceinline__ q4 q4zero() { return {0.0, 0.0, 0.0, 0.0}; }
+I2_HD __forceinline__ q4 operator+(q4 a, q4 b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+I2_HD __forceinline__ q4 operator-(q4 a, q4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+I2_HD __forceinline__ q4 operator*(double s, q4 a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
 
 // Per-panel record: vertices A, B, C (reference vertex order), the edge
 // tangents of the inner integral (taua = (C-B)/|C-B|, taub = (A-C)/|A-C|,
@@ -69,13 +76,13 @@ static_assert(sizeof(Panel) == 192, "Panel record is 192 bytes");
 
 // ------------------------------------------------------------ helpers of cuda_math
 // src/common/cuda_math.cuh:31-49
-__device__ __forceinline__ double sign_db(double x) {
+I2_HD __forceinline__ double sign_db(double x) {
     if (fabs(x) < DOUBLE_MIN) return 0.0;
     return (x > DOUBLE_MIN) ? 1.0 : -1.0;
 }
-__device__ __forceinline__ double arg_db(double x) { return (x > DOUBLE_MIN) ? 0.0 : PI; }
+I2_HD __forceinline__ double arg_db(double x) { return (x > DOUBLE_MIN) ? 0.0 : PI; }
 // src/common/cuda_math.cu:14-27
-__device__ __forceinline__ double angle(d3 a, d3 b) {
+I2_HD __forceinline__ double angle(d3 a, d3 b) {
     const double den = sqrt(dot(a, a) * dot(b, b));
     if (den < EPS_ZERO) return 0.0;
     const double r = dot(a, b) / den;
@@ -87,7 +94,7 @@ __device__ __forceinline__ double angle(d3 a, d3 b) {
 // ------------------------------------------------------------ inner integral
 // evaluatorJ3DK.cu:266-313.  (Psi, Theta) of panel J at point p, with
 // Theta n_J + Psi x n_J = integral over K_J of (p - r)/|p - r|^3.
-__device__ __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, const d3& ta, const d3& tb, const d3& tc) {
+I2_HD __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, const d3& ta, const d3& tb, const d3& tc) {
     d3 ova = p - A, ovb = p - B, ovc = p - C;
     const double lva = vlen(ova), lvb = vlen(ovb), lvc = vlen(ovc);
     ova = divs(ova, lva); ovb = divs(ovb, lvb); ovc = divs(ovc, lvc);
@@ -109,12 +116,46 @@ __device__ __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3
     return r;
 }
 
+// Same (Psi, Theta) without unit vectors, for the fast path.  With
+// d = p - vertex and l = |d|, the reference's l (1 + (d/l).tau) is l + d.tau,
+// and the solid-angle atan2 has both arguments multiplied by la lb lc > 0
+// (Van Oosterom and Strackee, IEEE Trans. Biomed. Eng. 30 (1983) 125-126):
+// identical in exact arithmetic, with 3 square roots, 1 division, 3 logs and
+// 1 atan2 per point instead of 13 divisions (log and atan2 from i2_math.hpp,
+// within 2 ulp).  The degenerate-branch tests
+// |1 + (d/l).tau| < eps become |l + d.tau| < eps l.
+struct JGeom { d3 a, b, c, ta, tb, tc; };
+I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J) {
+    const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
+    const double la = sqrt(dot(da, da)), lb = sqrt(dot(db, db)), lc = sqrt(dot(dc, dc));
+    double n1 = la + dot(da, J.tc), e1 = lb + dot(db, J.tc);
+    double n2 = lb + dot(db, J.ta), e2 = lc + dot(dc, J.ta);
+    double n3 = lc + dot(dc, J.tb), e3 = la + dot(da, J.tb);
+    const double h = 0.5 * EPS_PSI_THETA2;
+    if (fabs(e1) < h * lb) { n1 = lb; e1 = la; }
+    if (fabs(e2) < h * lc) { n2 = lc; e2 = lb; }
+    if (fabs(e3) < h * la) { n3 = la; e3 = lc; }
+    const double e12 = e1 * e2;
+    const double inv = 1.0 / (e12 * e3);
+    const double t1 = i2m::log_pos(n1 * (e2 * e3) * inv);
+    const double t2 = i2m::log_pos(n2 * (e1 * e3) * inv);
+    const double t3 = i2m::log_pos(n3 * e12 * inv);
+    q4 r;
+    r.x = t1 * J.tc.x + t2 * J.ta.x + t3 * J.tb.x;
+    r.y = t1 * J.tc.y + t2 * J.ta.y + t3 * J.tb.y;
+    r.z = t1 * J.tc.z + t2 * J.ta.z + t3 * J.tb.z;
+    const double y = dot(cross(da, db), dc);
+    const double x = la * lb * lc + dot(da, db) * lc + dot(db, dc) * la + dot(dc, da) * lb;
+    r.w = 2.0 * i2m::atan2_fast(y, x);
+    return r;
+}
+
 // ------------------------------------------------------------ singular parts
 // Edge-sharing pair: vertices of K_j rotated so that JA is off the shared
 // edge (evaluatorJ3DK.cu:797-847), plus the tangents and 1/|JC-JB| that
 // evaluatorJ3DK.cu:338-343 recompute per point.
 struct EdgeCtx { d3 JB, JC, taua, taub, tauc; double ilvc; };
-__device__ __forceinline__ EdgeCtx edge_ctx(d3 JA, d3 JB, d3 JC) {
+I2_HD __forceinline__ EdgeCtx edge_ctx(d3 JA, d3 JB, d3 JC) {
     EdgeCtx c;
     c.JB = JB; c.JC = JC;
     c.taua = unit(JA - JC);
@@ -125,7 +166,7 @@ __device__ __forceinline__ EdgeCtx edge_ctx(d3 JA, d3 JB, d3 JC) {
     return c;
 }
 // evaluatorJ3DK.cu:315-350
-__device__ __forceinline__ q4 sing_edge(d3 p, const EdgeCtx& c) {
+I2_HD __forceinline__ q4 sing_edge(d3 p, const EdgeCtx& c) {
     d3 va = p - c.JB, vb = p - c.JC;
     const double lva = vlen(va), lvb = vlen(vb);
     va = divs(va, lva); vb = divs(vb, lvb);
@@ -144,7 +185,7 @@ __device__ __forceinline__ q4 sing_edge(d3 p, const EdgeCtx& c) {
 // Vertex-sharing pair: JA is the shared vertex; e and the two delta angles
 // follow evaluatorJ3DK.cu:376-399 (two sign flips of e).
 struct VertexCtx { d3 JA, taua, taub, e; double invAri; };
-__device__ __forceinline__ d3 vertex_e(d3 nI, d3 nJ, d3 taua, d3 taub, double* dA, double* dB) {
+I2_HD __forceinline__ d3 vertex_e(d3 nI, d3 nJ, d3 taua, d3 taub, double* dA, double* dB) {
     d3 e = cross(nI, nJ);
     if (dot(e, e) < EPS_ZERO2) e = taub;
     else e = unit(e);
@@ -163,7 +204,7 @@ __device__ __forceinline__ d3 vertex_e(d3 nI, d3 nJ, d3 taua, d3 taub, double* d
     *dA = dx; *dB = dy;
     return e;
 }
-__device__ __forceinline__ VertexCtx vertex_ctx(d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
+I2_HD __forceinline__ VertexCtx vertex_ctx(d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
     VertexCtx c;
     c.JA = JA;
     c.taua = unit(JA - JC);
@@ -174,7 +215,7 @@ __device__ __forceinline__ VertexCtx vertex_ctx(d3 JA, d3 JB, d3 JC, d3 nI, d3 n
     return c;
 }
 // evaluatorJ3DK.cu:352-407
-__device__ __forceinline__ q4 sing_vertex(d3 p, const VertexCtx& c) {
+I2_HD __forceinline__ q4 sing_vertex(d3 p, const VertexCtx& c) {
     d3 ovc = p - c.JA;
     const double lvc = vlen(ovc);
     ovc = divs(ovc, lvc);
@@ -190,10 +231,10 @@ __device__ __forceinline__ q4 sing_vertex(d3 p, const VertexCtx& c) {
 }
 
 // ------------------------------------------------------------ topology of a pair
-__device__ __forceinline__ int vid(const Panel& P, int k) { return k == 0 ? P.v0 : (k == 1 ? P.v1 : P.v2); }
-__device__ __forceinline__ d3 vtx(const Panel& P, int k) { return k == 0 ? P.a : (k == 1 ? P.b : P.c); }
+I2_HD __forceinline__ int vid(const Panel& P, int k) { return k == 0 ? P.v0 : (k == 1 ? P.v1 : P.v2); }
+I2_HD __forceinline__ d3 vtx(const Panel& P, int k) { return k == 0 ? P.a : (k == 1 ? P.b : P.c); }
 // number of shared vertex indices (src/Mesh3d.cu:111-122)
-__device__ __forceinline__ int shared_count(const Panel& I, const Panel& J) {
+I2_HD __forceinline__ int shared_count(const Panel& I, const Panel& J) {
     int c = 0;
     if (I.v0 == J.v0 || I.v0 == J.v1 || I.v0 == J.v2) ++c;
     if (I.v1 == J.v0 || I.v1 == J.v1 || I.v1 == J.v2) ++c;
@@ -201,7 +242,7 @@ __device__ __forceinline__ int shared_count(const Panel& I, const Panel& J) {
     return c;
 }
 // evaluatorJ3DK.cu:782-795: position of the shared vertex (last match wins)
-__device__ __forceinline__ void shifts_vertex(const Panel& I, const Panel& J, int* s1, int* s2) {
+I2_HD __forceinline__ void shifts_vertex(const Panel& I, const Panel& J, int* s1, int* s2) {
     int a = 0, b = 0;
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
@@ -209,7 +250,7 @@ __device__ __forceinline__ void shifts_vertex(const Panel& I, const Panel& J, in
     *s1 = a; *s2 = b;
 }
 // evaluatorJ3DK.cu:797-825: position of the vertex off the shared edge
-__device__ __forceinline__ void shifts_edge(const Panel& I, const Panel& J, int* s1, int* s2) {
+I2_HD __forceinline__ void shifts_edge(const Panel& I, const Panel& J, int* s1, int* s2) {
     int mi = 0, mj = 0;   // bit masks of matched positions
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
@@ -220,11 +261,11 @@ __device__ __forceinline__ void shifts_edge(const Panel& I, const Panel& J, int*
 
 // ------------------------------------------------------------ analytic singular integrals
 // evaluatorJ3DK.cu:418-421
-__device__ __forceinline__ double phi_f(double sA, double cA, double sG, double cG, double sXi, double cosLambda) {
+I2_HD __forceinline__ double phi_f(double sA, double cA, double sG, double cG, double sXi, double cosLambda) {
     return 2.0 * atan2(sXi * sA * sG, 1.0 - cA + cG + cosLambda);
 }
 // evaluatorJ3DK.cu:438-461 -> (q_Theta, q_Psi)
-__device__ inline void q_theta_psi(double sA, double cA, double sB, double cB, double sG, double cG, double sNu, double cNu,
+I2_HD inline void q_theta_psi(double sA, double cA, double sB, double cB, double sG, double cG, double sNu, double cNu,
                                    double sXi, double cXi, double cosMu, double cosLambda, double* qT, double* qP) {
     const double phi1 = phi_f(sA, cA, sG, cG, sXi, cosLambda);
     const double phi2 = phi_f(sA, -cA, sG, -cG, sXi, cosLambda);
@@ -240,14 +281,14 @@ __device__ inline void q_theta_psi(double sA, double cA, double sB, double cB, d
                       phi2 * sXi * sG * sNu * sB);
 }
 // evaluatorJ3DK.cu:474-481 (coplanar case; q_Theta takes its documented value 0)
-__device__ inline void q_theta_psi_zero(double sB, double cB, double sNu, double cNu, double sinAlpha, double* qT, double* qP) {
+I2_HD inline void q_theta_psi_zero(double sB, double cB, double sNu, double cNu, double sinAlpha, double* qT, double* qP) {
     *qT = 0.0;
     *qP = 1.5 - (cNu * sB * log(1.0 + cNu) + sNu * cB * log(1.0 - cB) + sinAlpha - sB + sNu + sB * cNu * log(sB / sNu)) / sinAlpha;
 }
 
 // evaluatorJ3DK.cu:603-672: closed-form integral over K_i of the edge-sharing singular part.
 // (IA, IB, IC) and (JA, JB, JC) are rotated so that IA and JA are off the shared edge.
-__device__ inline q4 integrate_sing_edge(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
+I2_HD inline q4 integrate_sing_edge(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
     const d3 taua = unit(JA - JC), taub = unit(JB - JA), tauc = unit(JC - JB);
     const double alpha = angle(IA - IC, IB - IC);
     const double beta = angle(IC - IB, IA - IB);
@@ -280,7 +321,7 @@ __device__ inline q4 integrate_sing_edge(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 J
 }
 
 // evaluatorJ3DK.cu:511-601 -> (q^Theta, q^Psi) for vertex-sharing pairs
-__device__ inline void q_theta_psi_cont(double sXi, double cXi, double mu, double sMu, double cMu, double logSinMu, double logSinNu,
+I2_HD inline void q_theta_psi_cont(double sXi, double cXi, double mu, double sMu, double cMu, double logSinMu, double logSinNu,
                                         double psi, double sPsi, double cPsi, double nu, double sNu, double cNu, double kappa,
                                         double sinKappa, double sinNuPsi, double sinMuPsi, double delta, double sD, double cD,
                                         double cosLambda, double cosTheta, double cosEta, double cosSigma, double cosChi,
@@ -351,7 +392,7 @@ __device__ inline void q_theta_psi_cont(double sXi, double cXi, double mu, doubl
 
 // evaluatorJ3DK.cu:674-780: closed-form integral over K_i of the vertex-sharing
 // singular part.  IA and JA are the shared vertex.
-__device__ inline q4 integrate_sing_vertex(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
+I2_HD inline q4 integrate_sing_vertex(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
     const d3 taua = unit(JA - JC), taub = unit(JB - JA);
     double dA, dB;
     const d3 e = vertex_e(nI, nJ, taua, taub, &dA, &dB);
@@ -399,7 +440,7 @@ __device__ inline q4 integrate_sing_vertex(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3
 
 // ------------------------------------------------------------ finalize
 // evaluatorJ3DK.cu:224-241 (vertex-sharing, with the 2*pi*p wrap of Theta)
-__device__ __forceinline__ d3 finalize_vertex(q4 I, d3 nJ, double areaI) {
+I2_HD __forceinline__ d3 finalize_vertex(q4 I, d3 nJ, double areaI) {
     int p = 0;
     const double refTheta = TWO_PI * areaI;
     if (I.w > refTheta) p = -((int)trunc((I.w - refTheta) / (2.0 * refTheta)) + 1);
@@ -408,14 +449,14 @@ __device__ __forceinline__ d3 finalize_vertex(q4 I, d3 nJ, double areaI) {
     return RECIP_4PI * ((I.w + 2.0 * p * refTheta) * nJ + cross(psi, nJ));
 }
 // evaluatorJ3DK.cu:254-264
-__device__ __forceinline__ d3 finalize_plain(q4 I, d3 nJ) {
+I2_HD __forceinline__ d3 finalize_plain(q4 I, d3 nJ) {
     const d3 psi = {I.x, I.y, I.z};
     return RECIP_4PI * (I.w * nJ + cross(psi, nJ));
 }
 
 // ------------------------------------------------------------ Runge rule
 // src/common/cuda_math.cuh:270-300 and src/evaluators/evaluator3d.cu:83-92
-__device__ __forceinline__ double runge_criterion(q4 cur, q4 prev, double pow2p) {
+I2_HD __forceinline__ double runge_criterion(q4 cur, q4 prev, double pow2p) {
     const q4 num = cur - prev;
     const q4 den = pow2p * prev - cur;
     const double x = (fabs(num.x) < DOUBLE_MIN && fabs(den.x) < DOUBLE_MIN) ? 0.0 : num.x / den.x;
@@ -430,7 +471,7 @@ __device__ __forceinline__ double runge_criterion(q4 cur, q4 prev, double pow2p)
 // midpoint splits (src/NumericalIntegrator3d.cu:54-72): children
 // {m_AB, B, m_BC}, {m_BC, C, m_CA}, {m_CA, A, m_AB}, {m_BC, m_CA, m_AB}.
 // `path` holds the child digits, most significant split first.
-__device__ __forceinline__ void sub_panel(d3& a, d3& b, d3& c, int level, uint32_t path) {
+I2_HD __forceinline__ void sub_panel(d3& a, d3& b, d3& c, int level, uint32_t path) {
     for (int l = level - 1; l >= 0; --l) {
         const int ch = (path >> (2 * l)) & 3;
         const d3 mBC = 0.5 * (b + c), mCA = 0.5 * (c + a), mAB = 0.5 * (a + b);

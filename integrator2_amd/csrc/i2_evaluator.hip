@@ -11,6 +11,7 @@
 // group of lanes that walks its 4^k sub-panels by path digits and reduces in
 // a fixed order, so results are bitwise reproducible.
 #include "i2_internal.hpp"
+#include "i2_batch.hpp"
 
 #include <algorithm>
 #include <fstream>
@@ -216,9 +217,28 @@ static int launch_level(i2_evaluator* ev, int64_t nitems, const int32_t* list, c
     if (nitems == 0) return I2_OK;
     const int ll = lanes_log2_for(level);
     const int64_t threads = nitems << ll;
+    LaunchRecord rec{TYPE, level, nitems, nullptr, nullptr};
+    if (ev->profile) {
+        I2_HIP(hipEventCreate(&rec.e0));
+        I2_HIP(hipEventCreate(&rec.e1));
+        I2_HIP(hipEventRecord(rec.e0, ev->stream));
+    }
     hipLaunchKernelGGL((k_level<TYPE, IMPLICIT>), dim3(blocks_for(threads, 256)), dim3(256), 0, ev->stream, nitems, list, src,
                        ev->mesh->d_panels.p, ev->integ->rule, level, ll, out);
-    return check_launch("k_level");
+    I2_TRY(check_launch("k_level"));
+    if (ev->profile) {
+        I2_HIP(hipEventRecord(rec.e1, ev->stream));
+        ev->launches.push_back(rec);
+    }
+    return I2_OK;
+}
+
+static void clear_launches(i2_evaluator* ev) {
+    for (auto& r : ev->launches) {
+        if (r.e0) (void)hipEventDestroy(r.e0);
+        if (r.e1) (void)hipEventDestroy(r.e1);
+    }
+    ev->launches.clear();
 }
 
 template <int TYPE, bool IMPLICIT>
@@ -283,6 +303,7 @@ int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check) {
     if (!g.ok) return I2_ERR_HIP;
     ev->ran = false;
     ev->all_pairs = all_pairs;
+    clear_launches(ev);
     // types in the reference's order: vertex-sharing, edge-sharing, disjoint (evaluator3d.cu:173-186)
     for (int t = 0; t < 3; ++t) {
         TypeState& ts = ev->type[t];
@@ -414,6 +435,8 @@ void i2_evaluator_destroy(i2_evaluator* ev) {
             ts.tasks.reset(); ts.buf[0].reset(); ts.buf[1].reset(); ts.results.reset(); ts.errors.reset();
             ts.conv.reset(); ts.rest[0].reset(); ts.rest[1].reset();
         }
+        clear_launches(ev);
+        engine_release_evaluator(ev);
         ev->d_counter.reset();
         if (ev->h_counter) (void)hipHostFree(ev->h_counter);
         if (ev->ev0) (void)hipEventDestroy(ev->ev0);
@@ -434,7 +457,48 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
     d.implicit = true;
     d.n = 2 * m->pairs[I2_NOT_NEIGHBORS];
     d.tasks.reset();
-    return evaluator_run(ev, true, checkCorrectness != 0);
+    if (ev->path == 0) return evaluator_run(ev, true, checkCorrectness != 0);
+    // batched path: per-pair outputs in the same task order
+    ev->ran = false;
+    ev->all_pairs = true;
+    clear_launches(ev);
+    I2_TRY(engine_run_evaluator(ev));
+    for (int t = 0; t < 3; ++t) {
+        TypeState& ts = ev->type[t];
+        ts.has_errors = false;
+        if (checkCorrectness && ts.n > 0) {
+            const double4* ints;
+            const double* res;
+            const int8_t* conv;
+            engine_evaluator_outputs(ev, t, &ints, &res, &conv);
+            I2_TRY(ts.errors.ensure(ts.n));
+            const int64_t half = ts.n / 2;
+            hipLaunchKernelGGL(k_symmetry, dim3(blocks_for(half, 256)), dim3(256), 0, ev->stream, half, res, ts.errors.p);
+            I2_TRY(check_launch("k_symmetry"));
+            ts.has_errors = true;
+        }
+    }
+    I2_HIP(hipStreamSynchronize(ev->stream));
+    ev->ran = true;
+    return I2_OK;
+}
+
+int i2_evaluator_set_path(i2_evaluator* ev, int path) {
+    if (!ev || (path != 0 && path != 1)) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    ev->path = path;
+    return I2_OK;
+}
+
+int i2_evaluator_set_tuning(i2_evaluator* ev, int pairsPerWorkgroup, int pointsPerLane) {
+    if (!ev) { set_error("null evaluator"); return I2_ERR_INVALID; }
+    if (pairsPerWorkgroup < 1 || pairsPerWorkgroup > 4096 || (pairsPerWorkgroup & (pairsPerWorkgroup - 1)) != 0 ||
+        (pointsPerLane != 1 && pointsPerLane != 2 && pointsPerLane != 4 && pointsPerLane != 13)) {
+        set_error("invalid launch shape (%d pairs per workgroup, %d points per lane)", pairsPerWorkgroup, pointsPerLane);
+        return I2_ERR_INVALID;
+    }
+    ev->fast_P = pairsPerWorkgroup;
+    ev->fast_ppl = pointsPerLane;
+    return I2_OK;
 }
 
 int i2_evaluator_run_pairs(i2_evaluator* ev, const int32_t* simple3, int64_t nSimple, const int32_t* attached3, int64_t nAttached,
@@ -497,13 +561,17 @@ int i2_evaluator_get_results(const i2_evaluator* ev, int type, int32_t* tasks3, 
             I2_HIP(hipMemcpy(tasks3, ts.tasks.p, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost));
         }
     }
-    if (integrals4) I2_HIP(hipMemcpy(integrals4, ts.buf[ts.final_buf].p, sizeof(double4) * n, hipMemcpyDeviceToHost));
-    if (results3) I2_HIP(hipMemcpy(results3, ts.results.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+    const double4* d_ints = ts.buf[ts.final_buf].p;
+    const double* d_res = ts.results.p;
+    const int8_t* d_conv = ts.conv.p;
+    if (ev->path == 1 && ev->all_pairs) engine_evaluator_outputs(ev, type, &d_ints, &d_res, &d_conv);
+    if (integrals4) I2_HIP(hipMemcpy(integrals4, d_ints, sizeof(double4) * n, hipMemcpyDeviceToHost));
+    if (results3) I2_HIP(hipMemcpy(results3, d_res, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
     if (errors) {
         if (!ts.has_errors) { set_error("symmetry errors were not computed (runAllPairs with checkCorrectness)"); return I2_ERR_STATE; }
         I2_HIP(hipMemcpy(errors, ts.errors.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     }
-    if (convLevel) I2_HIP(hipMemcpy(convLevel, ts.conv.p, (size_t)n, hipMemcpyDeviceToHost));
+    if (convLevel) I2_HIP(hipMemcpy(convLevel, d_conv, (size_t)n, hipMemcpyDeviceToHost));
     return I2_OK;
 }
 
@@ -519,6 +587,38 @@ int i2_evaluator_type_stats(const i2_evaluator* ev, int type, double* ms, int64_
     if (!ev->ran) { set_error("no results: run the evaluator first"); return I2_ERR_STATE; }
     if (ms) *ms = ev->type[type].ms;
     if (refinedTasks) *refinedTasks = ev->type[type].refined_tasks;
+    return I2_OK;
+}
+
+int i2_evaluator_set_profiling(i2_evaluator* ev, int enable) {
+    if (!ev) { set_error("null evaluator"); return I2_ERR_INVALID; }
+    ev->profile = enable != 0;
+    return I2_OK;
+}
+
+int i2_evaluator_launch_records(const i2_evaluator* ev, int64_t capacity, int32_t* type_level, int64_t* tasks, double* ms, int64_t* count) {
+    if (!ev || !count) { set_error("null argument"); return I2_ERR_INVALID; }
+    *count = (int64_t)ev->launches.size();
+    const int64_t n = std::min<int64_t>(capacity, (int64_t)ev->launches.size());
+    DeviceGuard g(ev->mesh->device);
+    if (!g.ok) return I2_ERR_HIP;
+    for (int64_t q = 0; q < n; ++q) {
+        const LaunchRecord& r = ev->launches[q];
+        if (type_level) { type_level[2 * q] = r.type; type_level[2 * q + 1] = r.level; }
+        if (tasks) tasks[q] = r.tasks;
+        if (ms) {
+            float f = 0.f;
+            I2_HIP(hipEventElapsedTime(&f, r.e0, r.e1));
+            ms[q] = f;
+        }
+    }
+    return I2_OK;
+}
+
+int i2_device_synchronize(int device) {
+    DeviceGuard g(device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_HIP(hipDeviceSynchronize());
     return I2_OK;
 }
 

@@ -146,8 +146,20 @@ struct TypeState {
 
 }  // namespace i2
 
+// One timed kernel launch of the plain path (profiling mode).
+struct LaunchRecord {
+    int type, level;
+    int64_t tasks;
+    hipEvent_t e0, e1;
+};
+
 struct i2_evaluator {
     const i2_mesh* mesh = nullptr;
+    int path = 0;                  // 0: plain per-type launches, 1: batched persistent kernels
+    i2_batch* fast = nullptr;      // engine of the batched path (one mesh)
+    int fast_P = 64, fast_ppl = 13;
+    bool profile = false;
+    std::vector<LaunchRecord> launches;
     i2_integrator* integ = nullptr;
     hipStream_t stream = nullptr;
     bool ran = false;

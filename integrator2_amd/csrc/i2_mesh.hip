@@ -1,6 +1,7 @@
 // i2_mesh.hip -- Mesh3D behind the C ABI: file loading, upload, the panel
 // table kernel (normals, areas, edge tangents) and the neighbour lists.
 #include "i2_internal.hpp"
+#include "i2_math.hpp"
 
 #include <algorithm>
 #include <fstream>
@@ -136,6 +137,29 @@ using namespace i2;
 extern "C" {
 
 const char* i2_last_error(void) { return g_last_error.c_str(); }
+
+int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out) {
+    if (!x || !out || n < 0 || (fn == 1 && !y)) { set_error("null argument"); return I2_ERR_INVALID; }
+    for (int64_t q = 0; q < n; ++q) {
+        if (fn == 0) out[q] = i2m::log_pos(x[q]);
+        else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
+        else { set_error("unknown function %d", fn); return I2_ERR_INVALID; }
+    }
+    return I2_OK;
+}
+
+int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9, double* out4) {
+    if (!pts || !tri9 || !out4 || n < 0) { set_error("null argument"); return I2_ERR_INVALID; }
+    using namespace i2d;
+    const d3 A = {tri9[0], tri9[1], tri9[2]}, B = {tri9[3], tri9[4], tri9[5]}, C = {tri9[6], tri9[7], tri9[8]};
+    const JGeom J = {A, B, C, unit(C - B), unit(A - C), unit(B - A)};
+    for (int64_t q = 0; q < n; ++q) {
+        const d3 p = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
+        const q4 r = fast ? theta_psi_fast(p, J) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
+        out4[4 * q] = r.x; out4[4 * q + 1] = r.y; out4[4 * q + 2] = r.z; out4[4 * q + 3] = r.w;
+    }
+    return I2_OK;
+}
 
 int i2_device_count(int* n) {
     if (!n) { set_error("null pointer"); return I2_ERR_INVALID; }

@@ -159,3 +159,20 @@ def symmetry(results3):
 
 def normal_draw(seed, pair_id, stream):
     return lib().o_normal_draw(seed, pair_id, stream)
+
+
+def _task_noise(self, t, ij, rlevel, L12, w, order, results=False):
+    """Per-task FP64 rounding noise of the reference's formulas (double vs long
+    double), relative to the task's integrals, or to its J when results=True."""
+    ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
+    rlevel = np.ascontiguousarray(rlevel, np.int8)
+    L12 = np.ascontiguousarray(L12, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.zeros(len(ij))
+    out_j = np.zeros(len(ij))
+    lib().o_task_noise(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12), _ptr(w),
+                       ctypes.c_int(len(w)), ctypes.c_int(order), _ptr(out), _ptr(out_j))
+    return out_j if results else out
+
+
+Oracle.task_noise = _task_noise

@@ -238,6 +238,45 @@ void o_singular_integrals(void* h, int type, const int32_t* ij, int64_t n, doubl
     }
 }
 
+// FP64 rounding noise of the reference's formulas for whole tasks: the
+// reported value of task q (numerical part at level rlevel[q] plus the closed
+// singular part) evaluated in double and in long double; returns
+// ||I_double - I_longdouble||_inf / ||I_longdouble||_inf per task.  Used by the
+// parity gate for tasks whose quadrature points fall near the extension of an
+// edge of K_j (1 + cos -> 0 in thetaPsi) or whose closed forms are near their
+// removable singularities.
+void o_task_noise(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n, const double* L12, const double* w,
+                  int nPts, int order, double* out, double* out_j) {
+    const Mesh& m = *(Mesh*)h;
+    Rule R = make_rule(L12, w, nPts, order);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < n; ++q) {
+        const int i = ij[2 * q], j = ij[2 * q + 1];
+        TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
+        Q4<double> v = task_level_value(m, R, type, i, j, rlevel[q], g) + singular_integral(m, type, i, j, g);
+        TaskGeom<long double> gl = task_geom<long double>(m, i, j, nullptr, -1);
+        Q4<long double> vl = task_level_value(m, R, type, i, j, rlevel[q], gl) + singular_integral(m, type, i, j, gl);
+        const double d = std::max(std::max(std::fabs(v.x - (double)vl.x), std::fabs(v.y - (double)vl.y)),
+                                  std::max(std::fabs(v.z - (double)vl.z), std::fabs(v.w - (double)vl.w)));
+        const double s = std::max(std::max(std::fabs((double)vl.x), std::fabs((double)vl.y)),
+                                  std::max(std::fabs((double)vl.z), std::fabs((double)vl.w)));
+        out[q] = d / std::max(s, 1e-300);
+        if (out_j) {
+            // the same for J, with the vertex-sharing 2*pi*p wrap taken from the double value
+            int p = 0;
+            V3<double> J = (type == VERTEX_SHARING) ? finalize_vertex(v, g.nJ, g.areaI, &p) : finalize_plain(v, g.nJ);
+            V3<long double> Jl;
+            if (type == VERTEX_SHARING) {
+                const long double refT = (long double)TWO_PI * gl.areaI;
+                Jl = (long double)RECIPROCAL_FOUR_PI * ((vl.w + (long double)(2.0 * p) * refT) * gl.nJ + cross(vecof(vl), gl.nJ));
+            } else Jl = finalize_plain(vl, gl.nJ);
+            const double dj = std::max(std::fabs(J.x - (double)Jl.x), std::max(std::fabs(J.y - (double)Jl.y), std::fabs(J.z - (double)Jl.z)));
+            const double sj = std::max(std::fabs((double)Jl.x), std::max(std::fabs((double)Jl.y), std::fabs((double)Jl.z)));
+            out_j[q] = dj / std::max(sj, 1e-300);
+        }
+    }
+}
+
 // Level-L value of the numerical part of task (i, j): sum over the 4^L sub-panels.
 void o_level_value(void* h, int type, int i, int j, const double* L12, const double* w, int nPts, int order, int level, double* out4) {
     const Mesh& m = *(Mesh*)h;

@@ -1,0 +1,56 @@
+"""GPU parity of the batched persistent path: against the oracle and against
+the plain path, per-pair and row-sum outputs, several launch shapes."""
+import numpy as np
+import pytest
+
+from tests.helpers import rel_err_rows
+from tests.test_gpu_parity import check_auto, check_fixed, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("tuning", [(64, 13), (16, 1), (32, 2), (64, 4), (128, 13)])
+def test_batched_config1_vs_oracle(gpu, oracle_mod, tuning):
+    xyz, tri = gpu.icosphere(2)
+    check_fixed(gpu, oracle_mod, xyz, tri, 0, path=1, tuning=tuning, counts={0: 2820, 1: 960, 2: 98300})
+
+
+@pytest.mark.parametrize("quirk", [True, False])
+def test_batched_auto_vs_oracle(gpu, oracle_mod, quirk):
+    xyz, tri = gpu.icosphere(2, (1.0, 1.0, 1.0), 0.05, 3)
+    check_auto(gpu, oracle_mod, xyz, tri, 3, 1e-8, quirk, path=1)
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_batched_fixed_levels_vs_oracle(gpu, oracle_mod, level):
+    xyz, tri = gpu.icosphere(1, (2.0, 1.0, 0.7), 0.05, 5)
+    check_fixed(gpu, oracle_mod, xyz, tri, level, path=1)
+
+
+def test_batched_row_sums_two_meshes(gpu):
+    """Two meshes in one batch, row-sum output: rows of sum_j J_ij match the
+    per-pair results of the plain path summed per control panel."""
+    i2 = gpu
+    meshes = [i2.icosphere(1, (1.0, 2.0, 0.5), 0.05, 7), i2.icosphere(2, (1.5, 1.0, 1.0), 0.05, 8)]
+    b = i2.Batch()
+    keep = []
+    for xyz, tri in meshes:
+        m = i2.Mesh3D()
+        m.setData(xyz, tri)
+        m.prepareMesh()
+        keep.append(m)
+        b.addMesh(m)
+    it = i2.NumericalIntegrator3D(keep[0])
+    it.setErrorControl(3, 1e-8, True)
+    b.configure(it, i2.BATCH_ROW_SUMS)
+    b.run()
+    for k, (xyz, tri) in enumerate(meshes):
+        rows = b.rowSums(k)
+        _, out, _ = run_gpu(i2, xyz, tri, maxLevel=3, tol=1e-8, path=1, check=False)
+        ref = np.zeros((len(tri), 4))
+        for t in range(3):
+            r = out[t]
+            np.add.at(ref, r["tasks"][:, 0], np.concatenate([r["results"], r["integrals"][:, 3:4]], axis=1))
+        err = np.abs(rows - ref).max() / np.abs(ref).max()
+        print(f"mesh {k}: row-sum max diff / max |row| = {err:.3e}")
+        assert err <= 1e-12

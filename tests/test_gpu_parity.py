@@ -2,26 +2,12 @@
 import numpy as np
 import pytest
 
-from tests.helpers import rel_err_rows
+from tests.helpers import parity_gate, reported_levels
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-11   # norm-wise relative error per ordered pair
 
-
-def _gate(orc, t, tasks, gpu, cpu, label):
-    """Parity gate: 1e-11 norm-wise relative error per ordered pair, widened to
-    1e-11 + 4*nu for singular pairs whose closed-form part carries FP64
-    rounding noise nu (measured by the oracle as double vs long double)."""
-    nu = orc.fp64_noise(t, tasks, cpu)
-    e = rel_err_rows(gpu, cpu)
-    bound = TOL + 4.0 * nu
-    ill = int((nu > 1e-12).sum())
-    print(f"{label}: max rel err {e.max():.3e}, ill-conditioned pairs {ill}, worst e/bound {(e / bound).max():.3f}")
-    assert (e <= bound).all()
-
-
-def _run_gpu(i2, xyz, tri, fixed=None, maxLevel=5, tol=1e-5, quirk=True, check=True):
+def run_gpu(i2, xyz, tri, fixed=None, maxLevel=5, tol=1e-5, quirk=True, check=True, path=0, tuning=None):
     m = i2.Mesh3D()
     m.setData(xyz, tri)
     m.prepareMesh()
@@ -31,39 +17,50 @@ def _run_gpu(i2, xyz, tri, fixed=None, maxLevel=5, tol=1e-5, quirk=True, check=T
     else:
         it.setErrorControl(maxLevel, tol, quirk)
     ev = i2.EvaluatorJ3DK(m, it)
+    ev.setPath(path)
+    if tuning:
+        ev.setTuning(*tuning)
     ev.runAllPairs(check)
     out = {t: ev.results(t, errors=check) for t in range(3)}
     Ks = {t: ev.refinementIterations(t) for t in range(3)}
-    return m, out, Ks
+    return (m, it, ev), out, Ks
+
+
+def check_fixed(i2, oracle_mod, xyz, tri, level, path=0, tuning=None, counts=None):
+    _, out, Ks = run_gpu(i2, xyz, tri, fixed=level, path=path, tuning=tuning)
+    orc = oracle_mod.Oracle(xyz, tri)
+    for t in range(3):
+        r = out[t]
+        if counts:
+            assert len(r["tasks"]) == counts[t]
+        I4, J3, conv, K = orc.run_tasks(t, r["tasks"], i2.QF3D13_L12, i2.QF3D13_W, 7, mode=0, level=level)
+        rl = np.full(len(conv), level, np.int8)
+        parity_gate(orc, i2, t, r["tasks"], r["integrals"], I4, rl, f"path {path} {tuning} L{level} type {t} integrals")
+        parity_gate(orc, i2, t, r["tasks"], r["results"], J3, rl, f"path {path} {tuning} L{level} type {t} results", results=True)
+        assert np.abs(r["errors"] - oracle_mod.symmetry(J3)).max() <= 1e-9
+
+
+def check_auto(i2, oracle_mod, xyz, tri, maxLevel, tol, quirk, path=0):
+    _, out, Ks = run_gpu(i2, xyz, tri, maxLevel=maxLevel, tol=tol, quirk=quirk, path=path)
+    orc = oracle_mod.Oracle(xyz, tri)
+    for t in range(3):
+        r = out[t]
+        I4, J3, conv, K = orc.run_tasks(t, r["tasks"], i2.QF3D13_L12, i2.QF3D13_W, 7, mode=1, level=maxLevel, tol=tol, quirk=quirk)
+        assert K == Ks[t]
+        mism = np.nonzero(conv != r["conv"])[0]
+        assert len(mism) == 0, f"type {t}: {len(mism)} refinement decisions differ"
+        rl = reported_levels(conv, K, quirk)
+        parity_gate(orc, i2, t, r["tasks"], r["integrals"], I4, rl, f"path {path} quirk {quirk} type {t} K {K} integrals")
+        parity_gate(orc, i2, t, r["tasks"], r["results"], J3, rl, f"path {path} quirk {quirk} type {t} K {K} results",
+                    results=True)
 
 
 def test_config1_fixed_level0(gpu, oracle_mod):
-    i2 = gpu
-    xyz, tri = i2.icosphere(2)
-    m, out, Ks = _run_gpu(i2, xyz, tri, fixed=0)
-    orc = oracle_mod.Oracle(xyz, tri)
-    expect_counts = {0: 2820, 1: 960, 2: 98300}
-    for t in range(3):
-        r = out[t]
-        assert len(r["tasks"]) == expect_counts[t]
-        I4, J3, conv, K = orc.run_tasks(t, r["tasks"], i2.QF3D13_L12, i2.QF3D13_W, 7, mode=0, level=0)
-        _gate(orc, t, r["tasks"], r["integrals"], I4, f"type {t} integrals")
-        _gate(orc, t, r["tasks"], r["results"], J3, f"type {t} results")
-        d_cpu = oracle_mod.symmetry(J3)
-        assert np.abs(r["errors"] - d_cpu).max() <= 1e-9
+    xyz, tri = gpu.icosphere(2)
+    check_fixed(gpu, oracle_mod, xyz, tri, 0, counts={0: 2820, 1: 960, 2: 98300})
 
 
-def test_icosphere2_auto_refinement(gpu, oracle_mod):
-    i2 = gpu
-    xyz, tri = i2.icosphere(2, (1.0, 1.0, 1.0), 0.05, 3)
-    for quirk in (True, False):
-        m, out, Ks = _run_gpu(i2, xyz, tri, maxLevel=3, tol=1e-8, quirk=quirk)
-        orc = oracle_mod.Oracle(xyz, tri)
-        for t in range(3):
-            r = out[t]
-            I4, J3, conv, K = orc.run_tasks(t, r["tasks"], i2.QF3D13_L12, i2.QF3D13_W, 7, mode=1, level=3, tol=1e-8, quirk=quirk)
-            assert K == Ks[t]
-            mism = np.nonzero(conv != r["conv"])[0]
-            assert len(mism) == 0, f"type {t}: {len(mism)} refinement decisions differ"
-            _gate(orc, t, r["tasks"], r["integrals"], I4, f"quirk {quirk} type {t} K {K} integrals")
-            _gate(orc, t, r["tasks"], r["results"], J3, f"quirk {quirk} type {t} K {K} results")
+@pytest.mark.parametrize("quirk", [True, False])
+def test_icosphere2_auto_refinement(gpu, oracle_mod, quirk):
+    xyz, tri = gpu.icosphere(2, (1.0, 1.0, 1.0), 0.05, 3)
+    check_auto(gpu, oracle_mod, xyz, tri, 3, 1e-8, quirk)
