@@ -4,13 +4,16 @@ Workload (one "step"): all 26,209,280 ordered panel pairs of the jittered
 ellipsoid -- icosphere subdivided 4x (2,562 vertices, 5,120 triangles), semi-axes
 (3, 1, 0.5), vertices moved along the normal by N(0, (0.05 * mean edge)^2) from
 seed 0 -- integrated in FP64 with the 13-point rule and automatic Runge error
-control (up to 3 refinement levels, tolerance 1e-8), all three neighbour types,
-analytic singular parts and the final J, inputs resident in HBM.
+control (up to 3 refinement levels, tolerance 1e-8): all three neighbour types,
+analytic singular parts and the final J, per-pair outputs in the reference's
+task order, inputs resident in HBM.
 
-Prints one JSON line: value = ordered pair-integrals per second of the whole
-step.  `roofline` is for the dominant kernel (disjoint pairs, refinement level 1),
-timed with HIP events on the library's stream; `cpu_baseline` times the CPU
-oracle (FP64 OpenMP restatement) on a stratified sample of the same tasks.
+Prints one JSON line.  value = ordered pair-integrals per second of the batched
+persistent path; plain_value is the same step through the plain per-type,
+per-level launches.  `roofline` is for the dominant kernel (the disjoint-pair
+sweep, refinement levels 0 and 1), timed with HIP events on the library's
+stream.  `cpu_baseline` times the CPU oracle (FP64 OpenMP restatement of the
+reference) on a stratified sample of the same tasks.
 """
 import argparse
 import json
@@ -25,18 +28,19 @@ sys.path.insert(0, ROOT)
 
 import integrator2_amd as i2  # noqa: E402
 
-N_PAIRS_CONFIG2 = 26_209_280
-HBM_PEAK_GBS = 8000.0
+N_PAIRS = 26_209_280
+N_DISJOINT = 26_147_900
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic FP64 work of one quadrature point of the inner integral
-# (thetaPsi, evaluatorJ3DK.cu:266-313, with the edge tangents recomputed per
-# point as written, plus the Gauss point and the weighted sum): 188 basic
-# operations, 6 square roots, 3 logarithms and 1 atan2.  Transcendentals are
-# weighted by their gfx950 FP64 VALU instruction counts (TRANS_WEIGHTS).
+# Algorithmic FP64 work of one quadrature point of the inner integral as the
+# reference writes it (thetaPsi, evaluatorJ3DK.cu:266-313, edge tangents
+# recomputed per point, plus the Gauss point and the weighted sum): 188 basic
+# operations, 6 square roots, 3 logarithms and 1 atan2.  Transcendentals count
+# as the FP64 VALU instructions the HIP math library spends on them on gfx950
+# (single-call kernels, hipcc 7.2): sqrt 16, log 92, atan2 99.
 BASIC_OPS_PER_POINT = 188
 TRANS_PER_POINT = {"sqrt": 6, "log": 3, "atan2": 1}
-TRANS_WEIGHTS = {"sqrt": 8, "log": 22, "atan2": 40}
+TRANS_WEIGHTS = {"sqrt": 16, "log": 92, "atan2": 99}
 POINTS = 13
 
 
@@ -44,109 +48,119 @@ def flops_per_point():
     return BASIC_OPS_PER_POINT + sum(TRANS_PER_POINT[k] * TRANS_WEIGHTS[k] for k in TRANS_PER_POINT)
 
 
-def make_config2():
-    return i2.icosphere(4, (3.0, 1.0, 0.5), 0.05, 0)
+def make_mesh():
+    xyz, tri = i2.icosphere(4, (3.0, 1.0, 0.5), 0.05, 0)
+    mesh = i2.Mesh3D(0)
+    mesh.setData(xyz, tri)
+    mesh.prepareMesh()
+    return xyz, tri, mesh
 
 
-def run_plain(ev):
-    ev.runAllPairs(False)
+def time_steps(ev, steps, warmup):
+    for _ in range(warmup):
+        ev.runAllPairs(False)
+    i2.synchronize(0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.runAllPairs(False)
+    i2.synchronize(0)
+    return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(xyz, tri, conv_by_type, budget_s=20.0):
-    """Times the CPU oracle on a stratified sample of the step's ordered tasks
-    (all refinement included) and extrapolates to the full task mix."""
+def cpu_baseline(xyz, tri):
+    """CPU oracle on a stratified sample of the step's ordered tasks (refinement
+    included), extrapolated to the full task mix."""
     import oracle
     orc = oracle.Oracle(xyz, tri)
     threads = oracle.lib().o_max_threads()
     rng = np.random.default_rng(0)
     total_s = 0.0
-    sample_desc = []
+    desc = []
     for t in range(3):
         tasks = orc.ordered_tasks(t)
         n = len(tasks)
-        m = min(n, 4000 if t < 2 else 20000)
+        m = min(n, 6000 if t < 2 else 40000)
         idx = rng.choice(n, size=m, replace=False) if m < n else np.arange(n)
         t0 = time.perf_counter()
         orc.run_tasks(t, tasks[idx], i2.QF3D13_L12, i2.QF3D13_W, 7, mode=1, level=3, tol=1e-8)
-        dt = time.perf_counter() - t0
-        total_s += dt * n / m
-        sample_desc.append(f"{m}/{n} {i2.TYPE_NAMES[t]}")
-    return {"value": N_PAIRS_CONFIG2 / total_s, "unit": "pair-integrals/s", "cores": threads, "kind": "port",
-            "sample": "stratified random ordered tasks, full adaptive refinement: " + ", ".join(sample_desc)}
+        total_s += (time.perf_counter() - t0) * n / m
+        desc.append(f"{m} of {n} {i2.TYPE_NAMES[t]}")
+    return {"value": N_PAIRS / total_s, "unit": "pair-integrals/s", "cores": threads, "kind": "port",
+            "sample": "random ordered tasks with full adaptive refinement, extrapolated per type: " + ", ".join(desc)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pairs-per-workgroup", type=int, default=64)
+    ap.add_argument("--points-per-lane", type=int, default=13)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
         sys.exit(2)
     if i2.device_count() < 1:
         raise SystemExit("no HIP device")
 
-    xyz, tri = make_config2()
-    mesh = i2.Mesh3D(0)
-    mesh.setData(xyz, tri)
-    mesh.prepareMesh()
+    xyz, tri, mesh = make_mesh()
     integ = i2.NumericalIntegrator3D(mesh)
     integ.setErrorControl(3, 1e-8, True)
-    ev = i2.EvaluatorJ3DK(mesh, integ)
 
-    for _ in range(args.warmup):
-        run_plain(ev)
-    i2.synchronize(0)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        run_plain(ev)
-    i2.synchronize(0)
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = N_PAIRS_CONFIG2 / (elapsed / args.steps)
+    fast = i2.EvaluatorJ3DK(mesh, integ)
+    fast.setPath(1)
+    fast.setTuning(args.pairs_per_workgroup, args.points_per_lane)
+    t_fast = time_steps(fast, args.steps, args.warmup)
 
-    # dominant kernel: disjoint pairs at refinement level 1, timed with events
-    ev.setProfiling(True)
-    run_plain(ev)
-    recs = ev.launchRecords()
-    ev.setProfiling(False)
-    dom = max(recs, key=lambda r: r[3])
-    dom_type, dom_level, dom_tasks, dom_ms = dom
-    sub_panels = 4 ** dom_level
-    flops = dom_tasks * sub_panels * POINTS * flops_per_point()
-    achieved_tflops = flops / (dom_ms * 1e-3) / 1e12
-    # HBM traffic per launch (algorithmic): one double4 written per task, 192-byte
-    # panel records of both panels read once
-    traffic = dom_tasks * 32 + 2 * tri.shape[0] * 192
-    refined = {t: ev.typeStats(t)[1] for t in range(3)}
-    Ks = {t: ev.refinementIterations(t) for t in range(3)}
+    plain = i2.EvaluatorJ3DK(mesh, integ)
+    plain.setPath(0)
+    t_plain = time_steps(plain, max(1, args.steps // 2), 1)
+
+    # dominant kernel: the disjoint sweep (levels 0 and 1), HIP events on the library stream
+    fast.setProfiling(True)
+    sweep_ms = []
+    for _ in range(3):
+        fast.runAllPairs(False)
+        sweep_ms.append(fast.sweepMs())
+    fast.setProfiling(False)
+    sweep_ms = float(np.median(sweep_ms))
+    points = N_DISJOINT * POINTS * 5      # level 0 (13 points) and level 1 (4 x 13 points)
+    achieved = points * flops_per_point() / (sweep_ms * 1e-3) / 1e12
+    # algorithmic HBM bytes of the sweep: 192-byte panel records of both panels and
+    # the per-pair outputs (two double4 and the convergence level)
+    traffic_bytes = N_DISJOINT * (2 * 32 + 1) + 2 * tri.shape[0] * 192
+    Ks = {i2.TYPE_NAMES[t]: fast.refinementIterations(t) for t in range(3)}
 
     out = {
         "metric": "FP64 pair-integrals/s (<=1e-11 rel err) batched vs plain; adjoint/forward ratio",
-        "value": value,
+        "value": N_PAIRS / t_fast,
         "unit": "pair-integrals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * t_fast,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "jittered ellipsoid (3,1,0.5), icosphere-4: 5120 triangles, 26,209,280 ordered pairs, "
-                               "13-point rule, auto refinement <=3 levels at tol 1e-8",
-                   "path": "plain (per-type, per-level launches)",
-                   "refinement_iterations": Ks, "refined_tasks": refined},
-        "roofline": {"bound": "valu-fp64", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": f"k_level type {dom_type} level {dom_level}", "kernel_ms": dom_ms,
-                     "algorithmic_bytes": traffic, "flops_per_point": flops_per_point()},
+        "config": {"workload": "jittered ellipsoid (3, 1, 0.5), icosphere-4: 5,120 triangles, 26,209,280 ordered pairs, "
+                               "13-point rule, automatic refinement up to 3 levels at tol 1e-8",
+                   "path": "batched persistent kernels in one HIP graph",
+                   "pairs_per_workgroup": args.pairs_per_workgroup, "points_per_lane": args.points_per_lane,
+                   "refinement_iterations": Ks},
+        "plain_value": N_PAIRS / t_plain,
+        "plain_ms_per_step": 1e3 * t_plain,
+        "batched_over_plain": t_plain / t_fast,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_sweep_disjoint (levels 0 and 1 of all disjoint pairs)", "kernel_ms": sweep_ms,
+                     "flops_per_point": flops_per_point(), "points": points,
+                     "algorithmic_bytes": traffic_bytes},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(xyz, tri, None)
+        out["cpu_baseline"] = cpu_baseline(xyz, tri)
     print(json.dumps(out))
 
 

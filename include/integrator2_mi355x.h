@@ -52,6 +52,9 @@ int i2_device_count(int* n);
  * reference's expression (thetaPsi, src/evaluators/evaluatorJ3DK.cu:266-313),
  * fast = 1 the division-free form used by the batched kernels.  For checking. */
 int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9, double* out4);
+/* Host-side evaluation of the kernels' FP64 logarithm (fn = 0, of x) and
+ * two-argument arctangent (fn = 1, atan2(y, x)).  For checking. */
+int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out);
 
 /* ======================================================================
  * Mesh3D (src/Mesh3d.cuh:55-126, src/Mesh3d.cu)
@@ -133,6 +136,9 @@ int i2_evaluator_set_profiling(i2_evaluator* ev, int enable);
  * milliseconds; *count receives the number of records (capacity may be 0). */
 int i2_evaluator_launch_records(const i2_evaluator* ev, int64_t capacity, int32_t* type_level, int64_t* tasks,
                                 double* ms, int64_t* count);
+/* Batched path in profiling mode: device milliseconds of the disjoint-pair
+ * sweep kernel (levels 0 and 1) of the last run. */
+int i2_evaluator_sweep_ms(const i2_evaluator* ev, double* ms);
 /* hipDeviceSynchronize on `device`. */
 int i2_device_synchronize(int device);
 /* Evaluator3D::outputResultsToFile (src/evaluators/evaluator3d.cu:354-457):
@@ -176,6 +182,10 @@ int i2_batch_refinement_iterations(const i2_batch* b, int mesh, int type, int* K
 int i2_batch_level_counts(const i2_batch* b, int64_t* counts);
 /* Total ordered tasks of the batch. */
 int i2_batch_task_count(const i2_batch* b, int64_t* n);
+/* Profiling mode: runs the kernels without the graph, with events around the
+ * disjoint sweep; i2_batch_sweep_ms returns its device time. */
+int i2_batch_set_profiling(i2_batch* b, int enable);
+int i2_batch_sweep_ms(const i2_batch* b, double* ms);
 
 /* ======================================================================
  * Synthetic meshes (no counterpart in the reference, which loads files)

@@ -270,6 +270,11 @@ class EvaluatorJ3DK:
     def setProfiling(self, enable=True):
         _check(lib().i2_evaluator_set_profiling(self._h, ctypes.c_int(int(enable))))
 
+    def sweepMs(self):
+        ms = _D()
+        _check(lib().i2_evaluator_sweep_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
     def launchRecords(self):
         """List of (type, level, tasks, ms) of the last run's integration launches (profiling mode)."""
         cnt = _I64()
@@ -321,6 +326,14 @@ class Batch:
     def run(self):
         ms = _D()
         _check(lib().i2_batch_run(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def setProfiling(self, enable=True):
+        _check(lib().i2_batch_set_profiling(self._h, ctypes.c_int(int(enable))))
+
+    def sweepMs(self):
+        ms = _D()
+        _check(lib().i2_batch_sweep_ms(self._h, ctypes.byref(ms)))
         return ms.value
 
     def pairResults(self, mesh, t):

@@ -518,6 +518,7 @@ static int launch_sweep(const Engine& E, hipStream_t st) {
     }
 #undef I2_SWEEP
     I2_TRY(check_launch("k_sweep_disjoint"));
+    if (E.prof1) I2_HIP(hipEventRecord(E.prof1, st));
     if (E.args.singular_items > 0) {
         if (E.out == OUT_PER_PAIR) hipLaunchKernelGGL((k_sweep_singular<OUT_PER_PAIR>), dim3(E.deep_grid), dim3(256), 0, st, E.args);
         else hipLaunchKernelGGL((k_sweep_singular<OUT_ROW_SUMS>), dim3(E.deep_grid), dim3(256), 0, st, E.args);
@@ -555,6 +556,7 @@ int batch_enqueue(Engine& E, hipStream_t st) {
     I2_TRY(check_launch("k_zero_u64"));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, st, E.d_levelmask.p, (int64_t)E.d_levelmask.n);
     I2_TRY(check_launch("k_zero_u64"));
+    if (E.prof0) I2_HIP(hipEventRecord(E.prof0, st));
     I2_TRY(launch_sweep(E, st));
     for (int L = 2; L <= A.last_level; ++L) {
         if (A.fixed && L != A.last_level) continue;   // fixed mode queues tasks at its level only
@@ -709,6 +711,8 @@ struct i2_batch {
     std::vector<int> K;
     std::vector<unsigned long long> counts;
     bool ran = false;
+    bool profile = false;
+    double sweep_ms = 0.0;
 };
 
 namespace i2 {
@@ -776,6 +780,9 @@ static int batch_enqueue_all(i2_batch* b) {
 
 static int batch_capture(i2_batch* b) {
     drop_graph(b);
+    hipEvent_t p0 = b->E.prof0, p1 = b->E.prof1;
+    b->E.prof0 = b->E.prof1 = nullptr;
+    struct Restore { i2_batch* b; hipEvent_t p0, p1; ~Restore() { b->E.prof0 = p0; b->E.prof1 = p1; } } restore{b, p0, p1};
     I2_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int st = batch_enqueue_all(b);
     hipGraph_t g = nullptr;
@@ -799,7 +806,14 @@ static int batch_run(i2_batch* b, double* ms) {
             b->dirty = false;
         }
         I2_HIP(hipEventRecord(b->e0, b->stream));
-        I2_HIP(hipGraphLaunch(b->exec, b->stream));
+        if (b->profile) {
+            // same kernels, launched directly so that events can bracket the sweep
+            if (!b->E.prof0) I2_HIP(hipEventCreate(&b->E.prof0));
+            if (!b->E.prof1) I2_HIP(hipEventCreate(&b->E.prof1));
+            I2_TRY(batch_enqueue_all(b));
+        } else {
+            I2_HIP(hipGraphLaunch(b->exec, b->stream));
+        }
         I2_HIP(hipEventRecord(b->e1, b->stream));
         bool overflow = false;
         I2_TRY(engine_check_queues(b->E, b->stream, &overflow, &b->counts));
@@ -807,6 +821,11 @@ static int batch_run(i2_batch* b, double* ms) {
             float f = 0.f;
             I2_HIP(hipEventElapsedTime(&f, b->e0, b->e1));
             if (ms) *ms = f;
+            if (b->profile) {
+                float g = 0.f;
+                I2_HIP(hipEventElapsedTime(&g, b->E.prof0, b->E.prof1));
+                b->sweep_ms = g;
+            }
             b->K.resize(3 * b->meshes.size());
             I2_HIP(hipMemcpy(b->K.data(), b->E.d_K.p, sizeof(int) * b->K.size(), hipMemcpyDeviceToHost));
             b->ran = true;
@@ -844,8 +863,10 @@ int engine_run_evaluator(i2_evaluator* ev) {
     }
     if (b->P != ev->fast_P || b->ppl != ev->fast_ppl) { b->P = ev->fast_P; b->ppl = ev->fast_ppl; b->dirty = true; }
     b->out = OUT_PER_PAIR;
+    b->profile = ev->profile;
     double ms = 0.0;
     I2_TRY(batch_run(b, &ms));
+    ev->fast_sweep_ms = b->profile ? b->sweep_ms : 0.0;
     for (int t = 0; t < 3; ++t) {
         TypeState& ts = ev->type[t];
         ts.n = 2 * m->pairs[t];
@@ -869,6 +890,8 @@ void engine_release_evaluator(i2_evaluator* ev) {
     i2_batch* b = ev->fast;
     if (!b) return;
     drop_graph(b);
+    if (b->E.prof0) (void)hipEventDestroy(b->E.prof0);
+    if (b->E.prof1) (void)hipEventDestroy(b->E.prof1);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
     b->stream = nullptr;   // owned by the evaluator
@@ -910,6 +933,8 @@ void i2_batch_destroy(i2_batch* b) {
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         drop_graph(b);
         b->outs.clear();
+        if (b->E.prof0) (void)hipEventDestroy(b->E.prof0);
+        if (b->E.prof1) (void)hipEventDestroy(b->E.prof1);
         if (b->e0) (void)hipEventDestroy(b->e0);
         if (b->e1) (void)hipEventDestroy(b->e1);
         if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1006,6 +1031,19 @@ int i2_batch_level_counts(const i2_batch* b, int64_t* counts) {
         counts[t] = n;
         counts[3 + t] = (b->cfg.fixed && b->cfg.fixed_level == 0) || (!b->cfg.fixed && b->cfg.max_level == 0) ? 0 : n;
     }
+    return I2_OK;
+}
+
+int i2_batch_set_profiling(i2_batch* b, int enable) {
+    if (!b) { set_error("null batch"); return I2_ERR_INVALID; }
+    b->profile = enable != 0;
+    return I2_OK;
+}
+
+int i2_batch_sweep_ms(const i2_batch* b, double* ms) {
+    if (!b || !ms) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!b->ran || !b->profile) { set_error("no profiled run"); return I2_ERR_STATE; }
+    *ms = b->sweep_ms;
     return I2_OK;
 }
 

@@ -87,6 +87,8 @@ struct Engine {
     DBuf<unsigned long long> d_levelmask;
     DBuf<int> d_K;
     int64_t total_tasks = 0;
+    // profiling: events around the disjoint sweep (non-captured runs only)
+    hipEvent_t prof0 = nullptr, prof1 = nullptr;
 };
 
 int lanes_for_points_per_lane(int ppl);

@@ -596,6 +596,13 @@ int i2_evaluator_set_profiling(i2_evaluator* ev, int enable) {
     return I2_OK;
 }
 
+int i2_evaluator_sweep_ms(const i2_evaluator* ev, double* ms) {
+    if (!ev || !ms) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!ev->ran || ev->path != 1 || !ev->profile) { set_error("no profiled batched run"); return I2_ERR_STATE; }
+    *ms = ev->fast_sweep_ms;
+    return I2_OK;
+}
+
 int i2_evaluator_launch_records(const i2_evaluator* ev, int64_t capacity, int32_t* type_level, int64_t* tasks, double* ms, int64_t* count) {
     if (!ev || !count) { set_error("null argument"); return I2_ERR_INVALID; }
     *count = (int64_t)ev->launches.size();

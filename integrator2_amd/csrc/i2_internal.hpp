@@ -158,6 +158,7 @@ struct i2_evaluator {
     int path = 0;                  // 0: plain per-type launches, 1: batched persistent kernels
     i2_batch* fast = nullptr;      // engine of the batched path (one mesh)
     int fast_P = 64, fast_ppl = 13;
+    double fast_sweep_ms = 0.0;    // disjoint sweep time of the last profiled batched run
     bool profile = false;
     std::vector<LaunchRecord> launches;
     i2_integrator* integ = nullptr;
