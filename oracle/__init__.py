@@ -33,6 +33,7 @@ def lib():
         L.o_normal_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         L.o_loss_tasks.restype = ctypes.c_double
         L.o_now.restype = ctypes.c_double
+        L.o_singular_integral.restype = ctypes.c_uint
         _lib = L
     return _lib
 
@@ -111,10 +112,11 @@ class Oracle:
         lib().o_singular_part(self._h, ctypes.c_int(t), ctypes.c_int(i), ctypes.c_int(j), _ptr(pts), _I64(len(pts)), _ptr(out))
         return out
 
-    def singular_integral(self, t, i, j):
+    def singular_integral(self, t, i, j, with_branches=False):
+        """Closed-form singular part of task (i, j) [, special-case branch bit mask]."""
         out = np.zeros(4)
-        lib().o_singular_integral(self._h, ctypes.c_int(t), ctypes.c_int(i), ctypes.c_int(j), _ptr(out))
-        return out
+        br = lib().o_singular_integral(self._h, ctypes.c_int(t), ctypes.c_int(i), ctypes.c_int(j), _ptr(out))
+        return (out, br) if with_branches else out
 
     def singular_integrals(self, t, ij):
         """Closed-form singular parts of ordered tasks in double and in long double."""

@@ -212,12 +212,15 @@ void o_singular_part(void* h, int type, int i, int j, const double* pts, int64_t
         out4[4 * q] = r.x; out4[4 * q + 1] = r.y; out4[4 * q + 2] = r.z; out4[4 * q + 3] = r.w;
     }
 }
-// Closed-form integral over K_i of the singular part of task (i, j).
-void o_singular_integral(void* h, int type, int i, int j, double* out4) {
+// Closed-form integral over K_i of the singular part of task (i, j); returns
+// the special-case branches it took (BR_* bit mask).
+unsigned o_singular_integral(void* h, int type, int i, int j, double* out4) {
     const Mesh& m = *(Mesh*)h;
+    branch_flags() = 0;
     TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
     Q4<double> r = singular_integral(m, type, i, j, g);
     out4[0] = r.x; out4[1] = r.y; out4[2] = r.z; out4[3] = r.w;
+    return branch_flags();
 }
 // Closed-form singular integrals of ordered tasks evaluated twice: in double
 // (the reference's precision) and in long double (x87, 64-bit significand).

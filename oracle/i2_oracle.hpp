@@ -25,6 +25,22 @@ namespace i2o {
 // ---------------------------------------------------------------- constants
 // src/common/constants.h:13-24
 constexpr double DOUBLE_MIN = 2e-6;
+
+// Special-case branches of the closed-form singular integrals taken by the
+// last evaluation on this thread (bit mask, for the pinning tests): the
+// reference's special cases are the limits of the general formulas and are
+// exact only at the limit, so within their 1e-6 bands they differ from the
+// true integral by up to ~1e-7 relative.
+enum : unsigned {
+    BR_CONT_XI_SIGMA = 1u,     // q_thetaPsi_cont: sin xi ~ 0 and cos sigma ~ +-1
+    BR_CONT_XI = 2u,           // q_thetaPsi_cont: sin xi ~ 0
+    BR_CONT_PSI = 4u,          // q_thetaPsi_cont: sin psi ~ 0, delta away from 0
+    BR_CONT_PSI_DELTA = 8u,    // q_thetaPsi_cont: sin psi ~ 0 and delta ~ 0
+    BR_VERTEX_THETA0 = 16u,    // vertex-sharing Theta set to 0 (|xi| < 1e-6)
+    BR_EDGE_ZERO = 32u,        // edge-sharing q_thetaPsi_zero (coplanar, beta ~ gamma or alpha ~ delta)
+    BR_VERTEX_E_TAUB = 64u,    // e := tau_b (parallel panels)
+};
+inline unsigned& branch_flags() { static thread_local unsigned f = 0; return f; }
 constexpr double EPS_ZERO = 1e-6;
 constexpr double EPS_ZERO2 = 1e-10;
 constexpr double EPS_PSI_THETA = EPS_ZERO;
@@ -230,7 +246,7 @@ template <class T> struct VertexCtx {
 template <class T>
 V3<T> vertex_e(const V3<T>& nI, const V3<T>& nJ, const V3<T>& taua, const V3<T>& taub, T* dA = nullptr, T* dB = nullptr) {
     V3<T> e = cross(nI, nJ);
-    if (val(vlen2(e)) < EPS_ZERO2) e = taub;
+    if (val(vlen2(e)) < EPS_ZERO2) { e = taub; branch_flags() |= BR_VERTEX_E_TAUB; }
     else e = normalize(e);
     auto deltas = [&](const V3<T>& ee, T* x, T* y) {
         *x = atan2(dot(cross(taua, ee), nJ), -dot(ee, taua));
@@ -347,6 +363,7 @@ S2<T> q_theta_psi_cont(S2<T> Xi, T mu, S2<T> Mu, T logSinMu, T logSinNu, T psi, 
     const double mulDelta = sign_db(Xi.s * Delta.s);
 
     if ((fabs(val(Xi.s)) < EPS_ZERO) && (1.0 - fabs(val(cosSigma)) < 0.5 * EPS_ZERO2) && (fabs(val(Psi.s)) > EPS_ZERO)) {
+        branch_flags() |= BR_CONT_XI_SIGMA;
         const double ara = arg_db(sin(T(0.5) * (nu + psi)) * cosSigma);
         const double arb = arg_db(cos(T(0.5) * (mu - psi)) * cosSigma);
         r.s = T(2.0 * mulPsi) * cosSigma * Xi.c / (sinKappa * Psi.s) * (Mu.s * sinNuPsi * T(ara) - Nu.s * sinMuPsi * T(arb));
@@ -355,6 +372,7 @@ S2<T> q_theta_psi_cont(S2<T> Xi, T mu, S2<T> Mu, T logSinMu, T logSinNu, T psi, 
         return r;
     }
     if ((fabs(val(Xi.s)) < EPS_ZERO) && (fabs(val(Psi.s)) > EPS_ZERO)) {
+        branch_flags() |= BR_CONT_XI;
         const double are = arg_db(sin(T(0.5) * (nu + mu)) + sin(T(0.5) * (mu - nu) - psi + delta * Xi.c));
         const double arc = arg_db(T(1.0) / tan(T(0.5) * (nu + psi)) + tan(T(0.5) * delta) * Xi.c);
         const double ard = arg_db(tan(T(0.5) * (mu - psi)) + tan(T(0.5) * delta) * Xi.c);
@@ -365,6 +383,7 @@ S2<T> q_theta_psi_cont(S2<T> Xi, T mu, S2<T> Mu, T logSinMu, T logSinNu, T psi, 
     }
     if (fabs(val(sin(psi))) < EPS_ZERO) {
         if (fabs(val(delta)) > EPS_ZERO) {
+            branch_flags() |= BR_CONT_PSI;
             r.s = T(2.0) * (Mu.s * Nu.s * (W * Delta.c * Xi.c - T(0.5) * (Lambda1 - Lambda2 * cosSigma) * Xi.s) / Delta.s +
                             (Anu + T(mulDelta * arg_db(sin(T(0.5) * (nu + psi))))) * Mu.s * Nu.c +
                             (Amu + T(mulDelta * arg_db(cos(T(0.5) * (mu - psi))))) * Mu.c * Nu.s) / sinKappa;
@@ -372,6 +391,7 @@ S2<T> q_theta_psi_cont(S2<T> Xi, T mu, S2<T> Mu, T logSinMu, T logSinNu, T psi, 
                   Mu.s * Nu.s * ((Lambda1 * Delta.c - Lambda2 * Psi.c) * Xi.c + T(2.0) * W * Xi.s) / (sinKappa * Delta.s);
             return r;
         } else {
+            branch_flags() |= BR_CONT_PSI_DELTA;
             r.s = T(2.0 * arg_db(Psi.c));
             r.c = T(0.5 * (1.0 - LOG2)) - T(0.5) * log((T(1.0) - Psi.c * Mu.c) * (T(1.0) + Psi.c * Nu.c) / sinKappa) +
                   (T(0.5) * sin(mu - nu) * Lambda1 + Psi.c * (Mu.s - Nu.s)) / sinKappa;
@@ -410,9 +430,9 @@ Q4<T> integrate_sing_edge(const PairVerts<T>& p, const V3<T>& nI, const V3<T>& n
                   N.s * log(tan(T(0.5) * beta) * tan(T(0.5) * nu)) / A.s +
                   log(tan(T(0.5) * alpha) * tan(T(0.5) * beta));
     S2<T> qA, qB;
-    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(beta) - val(gamma)) < EPS_ZERO)) qA = q_theta_psi_zero(B, N, A.s);
+    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(beta) - val(gamma)) < EPS_ZERO)) { qA = q_theta_psi_zero(B, N, A.s); branch_flags() |= BR_EDGE_ZERO; }
     else qA = q_theta_psi(A, B, G, N, X, cosMu, cosLambda);
-    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(alpha) - val(delta)) < EPS_ZERO)) qB = q_theta_psi_zero(A, N, B.s);
+    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(alpha) - val(delta)) < EPS_ZERO)) { qB = q_theta_psi_zero(A, N, B.s); branch_flags() |= BR_EDGE_ZERO; }
     else qB = q_theta_psi(B, A, D, N, X, cosSigma, cosTheta);
     Q4<T> r = vecpart(areaI * (qA.c * taub + qB.c * taua - qab * tauc));
     r.w = areaI * (qA.s + qB.s);
@@ -457,6 +477,7 @@ Q4<T> integrate_sing_vertex(const PairVerts<T>& p, const V3<T>& nI, const V3<T>&
     const S2<T> qB = q_theta_psi_cont(X, mu, Mu, logSinMu, logSinNu, psi, Ps, nu, Nu, kappa, sinKappa,
                                       NuPsi.s, MuPsi.s, dB, DB, lB, tB, eB, sB, cB_);
     Q4<T> r = vecpart(areaI * (qA.c * taua + qB.c * taub));
+    if (fabs(val(xi)) < EPS_ZERO) branch_flags() |= BR_VERTEX_THETA0;
     r.w = (fabs(val(xi)) < EPS_ZERO) ? T(0.0) : (areaI * (qA.s - qB.s));
     return r;
 }

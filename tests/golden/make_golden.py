@@ -1,0 +1,223 @@
+"""Writes the golden fixtures under tests/golden/ (run in a checkout that has
+the reference sources at /root/reference; the fixtures are data and travel
+without them).
+
+quadrature_rules.json
+    The decimal literals of the reference's triangle rules qf3D1 ... qf3D13
+    (src/QuadratureFormula3d.cuh), kept as strings so tests can demand exact
+    equality with the package's tables.
+
+two_triangle_cases.json
+    The two-triangle meshes of the reference's examples/ folder (Case-*.dat,
+    Case1*.dat, G1Sosed.dat, G1new.dat, G1contact*.dat, G1Cont.dat), read the way
+    Mesh3D::loadMeshFromFile reads them (src/Mesh3d.cu:159-217: declared counts,
+    1-based indices, only cells of type 203).  For each ordered pair (i, j) of
+    the two cells it stores two integrals over K_i computed by brute-force
+    double-exponential (tanh-sinh) cubature, independent of the 13-point rule
+    and of the refinement loop:
+      total    = int_{K_i} thetaPsi(M; K_j) dM       (the pair's exact (Psi, Theta))
+      singular = int_{K_i} s_ij(M) dM                (the subtracted singular part)
+    and, for edge-sharing pairs, whether the two cells traverse the shared edge
+    in opposite directions (consistent orientation).
+    thetaPsi and the edge-sharing s_ij are evaluated here in well-conditioned
+    forms of the reference's formulas (near the shared edge the reference's
+    1 + cos terms cancel); the vertex-sharing s_ij is the oracle's point
+    function, whose arguments stay well-conditioned near the shared vertex.
+    The cubature uses the collapsed map of K_i onto the unit square with the
+    singular vertex or edge at an end of both variables, where tanh-sinh
+    converges exponentially; `cubature_err` is the difference between two step
+    sizes.
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+REF = "/root/reference"
+
+import oracle  # noqa: E402
+
+
+def parse_rules():
+    src = open(os.path.join(REF, "src", "QuadratureFormula3d.cuh")).read()
+    rules = {}
+    for m in re.finditer(r"const\s+QuadratureFormula3D\s+(qf3D\d+)\s*\{(.*?)\n\};", src, re.S):
+        name, body = m.group(1), m.group(2)
+        num = r"[-+]?\d*\.\d*(?:[eE][-+]?\d+)?|[-+]?\d+"
+        # body: { {x, y}, ... }, { w, ... }, order
+        coords_txt, rest = body.split("},\s*{", 1) if False else (None, None)
+        pairs = re.findall(r"\{\s*(" + num + r")\s*,\s*(" + num + r")\s*\}", body)
+        last_brace = body.rfind("}")
+        order = int(re.findall(r"\d+", body[last_brace:])[0])
+        wblock = body[:last_brace]
+        wstart = wblock.rfind("{")
+        weights = re.findall(num, wblock[wstart + 1:])
+        rules[name] = {"coordinates": [[a, b] for a, b in pairs], "weights": weights, "order": order}
+        assert len(pairs) == len(weights), name
+    return rules
+
+
+def load_dat(path):
+    """Mesh3D::loadMeshFromFile semantics (src/Mesh3d.cu:159-217)."""
+    tok = open(path).read().split("\n")
+    lines = [l.split() for l in tok if l.strip()]
+    nV, nC = int(lines[0][0]), int(lines[0][1])
+    xyz = np.array([[float(x) for x in l[1:4]] for l in lines[1:1 + nV]])
+    cells = []
+    for l in lines[1 + nV:1 + nV + nC]:
+        if int(l[1]) == 203:
+            cells.append([int(l[2]) - 1, int(l[3]) - 1, int(l[4]) - 1])
+    return xyz, np.array(cells, np.int32)
+
+
+def ts_nodes(h, tmax=3.0):
+    """tanh-sinh nodes on [0, 1]: x, 1 - x (computed without cancellation), weights."""
+    t = np.arange(-tmax, tmax + 1e-12, h)
+    s = 0.5 * np.pi * np.sinh(t)
+    x = 1.0 / (1.0 + np.exp(-2.0 * s))
+    xm = 1.0 / (1.0 + np.exp(2.0 * s))
+    w = h * 0.5 * np.pi * np.cosh(t) / (2.0 * np.cosh(s) ** 2)
+    return x, xm, w
+
+
+def cubature(f, O, P, Q, h):
+    """int over triangle (O, P, Q) of f, singular at the corner O and along the
+    side OP: M = O + u((P - O) + v(Q - P)), Jacobian 2 * area * u, with the
+    caller's geometry translated so that O is the origin (points near O keep
+    full relative precision)."""
+    x, xm, w = ts_nodes(h)
+    U, V = np.meshgrid(np.arange(len(x)), np.arange(len(x)), indexing="ij")
+    u, v, wu, wv = x[U].ravel(), x[V].ravel(), w[U].ravel(), w[V].ravel()
+    area2 = np.linalg.norm(np.cross(P - O, Q - O))
+    pts = u[:, None] * (P - O) + (u * v)[:, None] * (Q - P)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        vals = f(pts)
+    wt = wu * wv * u * area2
+    # points closer to a line of K_j than the coordinates resolve (next to the
+    # corner, u v < 1e-25) evaluate to log(0); they carry a negligible weight
+    bad = ~np.isfinite(vals).all(axis=1)
+    assert wt[bad].sum() < 1e-15 * area2, wt[bad].sum()
+    return (np.where(bad[:, None], 0.0, vals) * wt[:, None]).sum(axis=0)
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _lpt(l, d, t):
+    """l + d.t for |d| = l, |t| = 1, without the cancellation of 1 + cos -> 0:
+    (l^2 - (d.t)^2) / (l - d.t) = |d x t|^2 / (l - d.t) when d.t < 0."""
+    dt = d @ t
+    c = np.cross(d, t)
+    alt = (c * c).sum(axis=-1) / np.where(dt < 0, l - dt, 1.0)
+    return np.where(dt < 0, alt, l + dt)
+
+
+def theta_psi_wc(p, A, B, C):
+    """The analytic inner integral (Psi, Theta) over triangle ABC at points p,
+    evaluated in a well-conditioned form: the same function as thetaPsi of
+    evaluatorJ3DK.cu:266-313, which loses digits within ~1e-8 of an edge line."""
+    da, db, dc = p - A, p - B, p - C
+    la, lb, lc = (np.linalg.norm(x, axis=1) for x in (da, db, dc))
+    tc, ta, tb = _unit(B - A), _unit(C - B), _unit(A - C)
+    t1 = np.log(_lpt(la, da, tc) / _lpt(lb, db, tc))
+    t2 = np.log(_lpt(lb, db, ta) / _lpt(lc, dc, ta))
+    t3 = np.log(_lpt(lc, dc, tb) / _lpt(la, da, tb))
+    psi = t1[:, None] * tc + t2[:, None] * ta + t3[:, None] * tb
+    y = (np.cross(da, db) * dc).sum(axis=1)
+    x = la * lb * lc + (da * db).sum(axis=1) * lc + (db * dc).sum(axis=1) * la + (dc * da).sum(axis=1) * lb
+    return np.concatenate([psi, (2.0 * np.arctan2(y, x))[:, None]], axis=1)
+
+
+def sing_edge_wc(p, JA, JB, JC):
+    """Edge-sharing singular part (evaluatorJ3DK.cu:315-350) for K_j rotated so
+    that JA is off the shared edge JB-JC, in a well-conditioned form."""
+    taua, taub, tauc = _unit(JA - JC), _unit(JB - JA), _unit(JC - JB)
+    ilvc = 1.0 / np.linalg.norm(JC - JB)
+    pa, pb = p - JB, p - JC
+    lva, lvb = np.linalg.norm(pa, axis=1), np.linalg.norm(pb, axis=1)
+    va, vb = pa / lva[:, None], pb / lvb[:, None]
+    l1 = np.log(_lpt(lvb, pb, -tauc) / _lpt(lva, pa, -tauc))
+    l2 = np.log(_lpt(lva, pa, taub) * ilvc)
+    l3 = np.log(_lpt(lvb, pb, -taua) * ilvc)
+    psi = l1[:, None] * tauc - l2[:, None] * taub - l3[:, None] * taua
+    th = 2.0 * (np.arctan2((np.cross(va, taub) * tauc).sum(axis=1), ((taub - tauc) * (taub + va)).sum(axis=1)) -
+                np.arctan2((np.cross(vb, taua) * tauc).sum(axis=1), ((taua - tauc) * (taua - vb)).sum(axis=1)))
+    return np.concatenate([psi, th[:, None]], axis=1)
+
+
+def pair_cubatures(orc_of, t, i, j, xyz, tri, h):
+    """(total, singular) over K_i for the ordered pair (i, j); orc_of(origin)
+    gives the oracle of the mesh translated by -origin."""
+    shared = [k for k in range(3) if tri[i][k] in set(tri[j])]
+    if t == 1:
+        # two halves of K_i split at the midpoint of the shared edge, each
+        # collapsed at one shared vertex with the shared edge along v = 0
+        apex = [k for k in range(3) if k not in shared][0]
+        pieces = [(shared[0], shared[1]), (shared[1], shared[0])]
+    else:
+        apex = shared[0]
+        pieces = [(apex, None)]
+    tot = np.zeros(4)
+    sing = np.zeros(4)
+    for (c, other) in pieces:
+        O = xyz[tri[i][c]]
+        X = xyz - O
+        Ti, Tj = X[tri[i]], X[tri[j]]
+        if t == 1:
+            P = 0.5 * (X[tri[i][c]] + X[tri[i][other]])
+            Q = Ti[apex]
+            off = [k for k in range(3) if tri[j][k] not in set(tri[i])][0]
+            JA, JB, JC = Tj[off], Tj[(off + 1) % 3], Tj[(off + 2) % 3]
+            sing_f = lambda p: sing_edge_wc(p, JA, JB, JC)
+        else:
+            P, Q = [Ti[k] for k in range(3) if k != apex]
+            orc = orc_of(O)
+            sing_f = lambda p: orc.singular_part(t, i, j, p)
+        tot += cubature(lambda p: theta_psi_wc(p, Tj[0], Tj[1], Tj[2]), np.zeros(3), P, Q, h)
+        sing += cubature(sing_f, np.zeros(3), P, Q, h)
+    return tot, sing
+
+
+def main():
+    rules = parse_rules()
+    with open(os.path.join(HERE, "quadrature_rules.json"), "w") as fh:
+        json.dump(rules, fh, indent=1)
+    ex = os.path.join(REF, "examples")
+    names = sorted(f for f in os.listdir(ex) if re.match(r"(Case.*|G1Sosed|G1new|G1contact.*|G1Cont)\.dat$", f))
+    cases = []
+    for name in names:
+        xyz, tri = load_dat(os.path.join(ex, name))
+        if len(tri) != 2:
+            continue
+        orc = oracle.Oracle(xyz, tri)
+        t = next(k for k in range(3) if orc.pair_count(k) == 1)
+        orc_of = lambda O, xyz=xyz, tri=tri: oracle.Oracle(xyz - O, tri)
+        entry = {"name": name, "xyz": xyz.tolist(), "tri": tri.tolist(), "type": t, "pairs": []}
+        if t == 1:
+            # a consistently oriented pair traverses the shared edge in opposite directions
+            e0 = {(int(tri[0][k]), int(tri[0][(k + 1) % 3])) for k in range(3)}
+            e1 = {(int(tri[1][k]), int(tri[1][(k + 1) % 3])) for k in range(3)}
+            entry["consistent_orientation"] = not (e0 & e1)
+        for (i, j) in ((0, 1), (1, 0)):
+            if t == 2:
+                continue
+            a_tot, a_sing = pair_cubatures(orc_of, t, i, j, xyz, tri, 1.0 / 24)
+            b_tot, b_sing = pair_cubatures(orc_of, t, i, j, xyz, tri, 1.0 / 48)
+            err = max(np.abs(a_tot - b_tot).max(), np.abs(a_sing - b_sing).max())
+            entry["pairs"].append({"i": i, "j": j, "total": b_tot.tolist(), "singular": b_sing.tolist(),
+                                   "cubature_err": float(err)})
+            print(f"{name} type {t} ({i},{j}) total {b_tot} sing {b_sing} err {err:.1e}")
+        cases.append(entry)
+    with open(os.path.join(HERE, "two_triangle_cases.json"), "w") as fh:
+        json.dump({"description": "reference examples/ two-triangle meshes; tanh-sinh cubatures over K_i",
+                   "cases": cases}, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

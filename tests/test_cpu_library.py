@@ -1,0 +1,184 @@
+"""CPU checks of the HIP library: it loads, exports every entry point the C
+header declares, its host-side math and mesh generation behave, and without a
+GPU it refuses to compute instead of falling back to the CPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "integrator2_mi355x.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return sorted(set(re.findall(r"\b(i2_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(i2):
+    lib = ctypes.CDLL(i2.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_library_was_built_for_gfx950(i2):
+    data = open(i2.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def _host_math(i2, fn, x, y=None):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    yp = None if y is None else np.ascontiguousarray(y, np.float64).ctypes.data_as(ctypes.c_void_p)
+    st = i2.lib().i2_host_math(ctypes.c_int(fn), x.ctypes.data_as(ctypes.c_void_p), yp,
+                               ctypes.c_int64(len(x)), out.ctypes.data_as(ctypes.c_void_p))
+    assert st == 0
+    return out
+
+
+def _ulp_err(got, ref_mp):
+    import mpmath
+    errs = []
+    for g, r in zip(got, ref_mp):
+        if r == 0:
+            errs.append(0.0 if g == 0 else np.inf)
+            continue
+        ulp = np.spacing(abs(float(r)))
+        errs.append(float(abs(mpmath.mpf(g) - r)) / ulp)
+    return np.array(errs)
+
+
+def test_host_log_within_2_ulp(i2):
+    import mpmath
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(0)
+    x = np.concatenate([np.exp(rng.uniform(-700, 700, 3000)), 1.0 + rng.uniform(-1e-3, 1e-3, 1000),
+                        [1.0, 2.0, 0.5, np.sqrt(0.5), np.sqrt(2.0), 2.2250738585072014e-308, 1.7976931348623157e308]])
+    got = _host_math(i2, 0, x)
+    e = _ulp_err(got, [mpmath.log(mpmath.mpf(v)) for v in x])
+    assert e.max() <= 2.0, e.max()
+    # outside the fast range: the library's log
+    special = _host_math(i2, 0, np.array([0.0, -1.0, np.inf, 5e-320]))
+    assert special[0] == -np.inf and np.isnan(special[1]) and special[2] == np.inf
+    assert abs(special[3] - np.log(5e-320)) < 1e-12
+
+
+def test_host_atan2_within_2_ulp(i2):
+    import mpmath
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(1)
+    y = rng.normal(size=4000) * np.exp(rng.uniform(-30, 30, 4000))
+    x = rng.normal(size=4000) * np.exp(rng.uniform(-30, 30, 4000))
+    # octant and centre boundaries
+    t = np.array([0.0, 0.19891236737965801, 0.41421356237309503, 0.66817863791929888, 1.0])
+    y = np.concatenate([y, t, -t, t, 1.0 / np.maximum(t, 1e-300)])
+    x = np.concatenate([x, np.ones(5), np.ones(5), -np.ones(5), np.ones(5)])
+    got = _host_math(i2, 1, x, y)
+    e = _ulp_err(got, [mpmath.atan2(mpmath.mpf(a), mpmath.mpf(b)) for a, b in zip(y, x)])
+    assert e.max() <= 2.0, e.max()
+    z = _host_math(i2, 1, np.array([1.0, -1.0, 0.0, 0.0]), np.array([0.0, 0.0, 0.0, -0.0]))
+    assert z[0] == 0.0 and z[1] == np.pi and z[2] == 0.0 and np.signbit(z[3])
+
+
+def _host_theta_psi(i2, fast, pts, tri9):
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    tri9 = np.ascontiguousarray(tri9, np.float64).reshape(9)
+    out = np.zeros((len(pts), 4))
+    st = i2.lib().i2_host_theta_psi(ctypes.c_int(fast), pts.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(len(pts)),
+                                    tri9.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    assert st == 0
+    return out
+
+
+def test_device_theta_psi_forms_match_oracle(i2, oracle_mod):
+    """The kernels' inner integral (the reference form and the division-free
+    form with the custom log/atan2), compiled for the host, against the
+    oracle's thetaPsi: norm-wise relative error 1e-11 for points whose
+    directions to the vertices stay 6 degrees away from the edge lines (near
+    those lines the reference's 1 + cos terms cancel).  The bound is the
+    reference formula's own conditioning: for far points Psi is a sum of
+    logarithms of ratios near 1, and the oracle itself is ~2e-12 from a
+    200-bit evaluation there."""
+    rng = np.random.default_rng(3)
+    orc = oracle_mod.Oracle(np.eye(3), np.array([[0, 1, 2]], np.int32))
+    worst = [0.0, 0.0]
+    for _ in range(50):
+        T = rng.normal(size=(3, 3))
+        pts = T.mean(axis=0) + 2.0 * rng.normal(size=(400, 3))
+        tau = [(T[(k + 1) % 3] - T[k]) / np.linalg.norm(T[(k + 1) % 3] - T[k]) for k in range(3)]
+        o = [(pts - v) / np.linalg.norm(pts - v, axis=1)[:, None] for v in T]
+        cos = np.stack([ov @ (sgn * t) for ov in o for t in tau for sgn in (1.0, -1.0)], axis=1)
+        pts = pts[(1.0 + cos).min(axis=1) > 5e-3]
+        ref = orc.theta_psi(pts, T.ravel())
+        for fast in (0, 1):
+            got = _host_theta_psi(i2, fast, pts, T.ravel())
+            e = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+            worst[fast] = max(worst[fast], e.max())
+    assert worst[0] <= 1e-11 and worst[1] <= 1e-11, worst
+
+
+def test_icosphere_generator(i2):
+    xyz, tri = i2.icosphere(2)
+    assert xyz.shape == (162, 3) and tri.shape == (320, 3)
+    assert np.allclose(np.linalg.norm(xyz, axis=1), 1.0, atol=1e-15)
+    # every edge shared by exactly two faces, traversed once in each direction
+    edges = {}
+    for t in tri:
+        for k in range(3):
+            e = (int(t[k]), int(t[(k + 1) % 3]))
+            assert e not in edges
+            edges[e] = True
+    assert all((b, a) in edges for (a, b) in edges)
+    # the ellipsoid map and the seeded normal jitter are deterministic
+    a1 = i2.icosphere(4, (3.0, 1.0, 0.5), 0.05, 0)
+    a2 = i2.icosphere(4, (3.0, 1.0, 0.5), 0.05, 0)
+    a3 = i2.icosphere(4, (3.0, 1.0, 0.5), 0.05, 1)
+    assert np.array_equal(a1[0], a2[0]) and not np.array_equal(a1[0], a3[0])
+    base = i2.icosphere(4, (3.0, 1.0, 0.5), 0.0, 0)[0]
+    d = a1[0] - base
+    # displacement along the ellipsoid normal (gradient of x^2/a^2 + ...)
+    g = base / np.array([9.0, 1.0, 0.25])
+    g /= np.linalg.norm(g, axis=1)[:, None]
+    assert np.abs(np.cross(d, g)).max() < 1e-12
+    edge = np.mean([np.linalg.norm(base[t[0]] - base[t[1]]) for t in a1[1]])
+    s = np.linalg.norm(d, axis=1).std() / edge
+    assert 0.02 < np.sqrt(((d * g).sum(axis=1) ** 2).mean()) / edge < 0.08, s
+
+
+def test_torus_generator(i2):
+    xyz, tri = i2.torus(16, 8, 1.0, 0.25, 0.0, 0)
+    assert xyz.shape == (128, 3) and tri.shape == (256, 3)
+    rho = np.hypot(xyz[:, 0], xyz[:, 1])
+    assert np.allclose(np.hypot(rho - 1.0, xyz[:, 2]), 0.25)
+    # outward normals: away from the tube's centre line
+    A, B, C = xyz[tri[:, 0]], xyz[tri[:, 1]], xyz[tri[:, 2]]
+    n = np.cross(B - A, C - A)
+    cen = (A + B + C) / 3
+    ring = np.stack([cen[:, 0], cen[:, 1], np.zeros(len(cen))], axis=1)
+    ring = ring / np.linalg.norm(ring, axis=1)[:, None]
+    assert ((n * (cen - ring)).sum(axis=1) > 0).all()
+
+
+def test_batch_spec(i2):
+    levels, axes = i2.batch_spec(64, 0)
+    assert levels.shape == (64,) and axes.shape == (64, 3)
+    assert set(np.unique(levels)) <= {2, 3, 4, 5} and len(np.unique(levels)) == 4
+    assert axes.min() >= 0.5 and axes.max() <= 3.0
+    l2, a2 = i2.batch_spec(64, 0)
+    assert np.array_equal(levels, l2) and np.array_equal(axes, a2)
+
+
+def test_product_refuses_without_device(i2):
+    """No CPU fallback: without a HIP device the mesh cannot be created and the
+    error names the cause."""
+    if i2.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(i2.I2Error) as ei:
+        i2.Mesh3D(0)
+    assert "device" in str(ei.value).lower()
