@@ -14,6 +14,11 @@ per-level launches.  `roofline` is for the dominant kernel (the disjoint-pair
 sweep, refinement levels 0 and 1), timed with HIP events on the library's
 stream.  `cpu_baseline` times the CPU oracle (FP64 OpenMP restatement of the
 reference) on a stratified sample of the same tasks.
+
+`adjoint` is the shape-sensitivity step on the jittered unit sphere of the
+same size (icosphere-4, axes (1, 1, 1), same jitter and refinement): the
+forward pass, then the loss L = sum w Theta + v . J over all ordered pairs
+and dL/dX by the adjoint kernels; `ratio` = adjoint time / forward time.
 """
 import argparse
 import json
@@ -65,6 +70,33 @@ def time_steps(ev, steps, warmup):
         ev.runAllPairs(False)
     i2.synchronize(0)
     return (time.perf_counter() - t0) / steps
+
+
+def adjoint_step(args):
+    """Forward (batched path) and adjoint on the 5,120-triangle jittered sphere."""
+    xyz, tri = i2.icosphere(4, (1.0, 1.0, 1.0), 0.05, 0)
+    mesh = i2.Mesh3D(0)
+    mesh.setData(xyz, tri)
+    mesh.prepareMesh()
+    integ = i2.NumericalIntegrator3D(mesh)
+    integ.setErrorControl(3, 1e-8, True)
+    ev = i2.EvaluatorJ3DK(mesh, integ)
+    ev.setPath(1)
+    ev.setTuning(args.pairs_per_workgroup, args.points_per_lane)
+    t_fwd = time_steps(ev, args.steps, args.warmup)
+    for _ in range(args.warmup):
+        ev.lossAndGrad(1)
+    reps = max(1, args.steps)
+    t0 = time.perf_counter()
+    adj_ms = []
+    for _ in range(reps):
+        L, G = ev.lossAndGrad(1)
+        adj_ms.append(ev.adjointMs())
+    t_adj = (time.perf_counter() - t0) / reps
+    return {"workload": "jittered unit sphere, icosphere-4 (5,120 triangles), refinement up to 3 levels at tol 1e-8; "
+                        "loss sum w Theta + v . J over 26,209,280 ordered pairs, weights N(0,1) from seed 1",
+            "forward_ms": 1e3 * t_fwd, "adjoint_ms": 1e3 * t_adj, "adjoint_device_ms": float(np.median(adj_ms)),
+            "ratio": t_adj / t_fwd, "loss": L, "grad_norm": float(np.linalg.norm(G))}
 
 
 def cpu_baseline(xyz, tri):
@@ -159,6 +191,7 @@ def main():
                      "flops_per_point": flops_per_point(), "points": points,
                      "algorithmic_bytes": traffic_bytes},
     }
+    out["adjoint"] = adjoint_step(args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(xyz, tri)
     print(json.dumps(out))

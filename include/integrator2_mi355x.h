@@ -155,6 +155,27 @@ int i2_evaluator_set_path(i2_evaluator* ev, int path);
  * per lane (1, 2, 4 or 13 = the whole 13-point rule in one lane). */
 int i2_evaluator_set_tuning(i2_evaluator* ev, int pairsPerWorkgroup, int pointsPerLane);
 
+/* ---- shape sensitivity (no counterpart in the reference) --------------
+ * Scalar loss over every ordered pair of the last runAllPairs,
+ *   L = sum_ij w_ij Theta_ij + v_ij . J_ij,
+ * Theta_ij the Theta channel of the pair's integrals, J_ij its result,
+ * w_ij ~ N(0,1), v_ij ~ N(0,1)^3 drawn from (seed, pair id i*nT + j), and
+ * dL/dX (3*nV doubles, vertex-major) by the hand-written adjoint kernels.
+ * Refinement levels, the vertex-sharing 2*pi*p wrap and all branches are
+ * frozen at the forward pass's choices.  Both integration paths. */
+int i2_evaluator_loss_and_grad(i2_evaluator* ev, uint64_t seed, double* loss, double* dLdX);
+/* Device milliseconds of the last adjoint run. */
+int i2_evaluator_adjoint_ms(const i2_evaluator* ev, double* ms);
+/* The loss weights (w, v) of n ordered pairs ij (2 ints each) of a mesh with nT cells. */
+int i2_host_loss_weights(uint64_t seed, const int32_t* ij, int64_t n, int nT, double* out4);
+/* Host-side reverse mode of one per-point function, for checking: kind 0
+ * thetaPsi (params: A, B, C, tau_a, tau_b, tau_c; 18), 1 edge-sharing singular
+ * part (JB, JC, tau_a, tau_b, tau_c, 1/|JC-JB|; 16), 2 vertex-sharing singular
+ * part (JA, tau_a, tau_b, e, 1/sqrt(S_i); 13).  Returns f(p) and gbar4 . df
+ * with respect to p (pbar3) and to every parameter (parbar). */
+int i2_host_point_adjoint(int kind, const double* p3, const double* params, const double* gbar4, double* f4,
+                          double* pbar3, double* parbar);
+
 /* ======================================================================
  * Batched path: many meshes, all neighbour types and refinement levels in one
  * captured HIP graph of persistent kernels fed by device work queues (no

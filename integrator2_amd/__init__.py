@@ -291,6 +291,20 @@ class EvaluatorJ3DK:
                                                        None if directory is None else directory.encode())
         return st == 0
 
+    def lossAndGrad(self, seed=1):
+        """Loss L = sum_ij w_ij Theta_ij + v_ij . J_ij over all ordered pairs of the last
+        runAllPairs (weights drawn from seed) and dL/dX, shape (nV, 3), from the adjoint kernels."""
+        nV = self.mesh.sizes()[0]
+        grad = np.zeros((nV, 3))
+        loss = ctypes.c_double()
+        _check(lib().i2_evaluator_loss_and_grad(self._h, ctypes.c_uint64(seed), ctypes.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
+    def adjointMs(self):
+        ms = ctypes.c_double()
+        _check(lib().i2_evaluator_adjoint_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
 
 # ---------------------------------------------------------------- batched path
 BATCH_PER_PAIR = 0

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libintegrator2_mi355x.so")
 
-SOURCES = ["i2_mesh.hip", "i2_evaluator.hip", "i2_batch.hip", "i2_meshgen.cpp"]
+SOURCES = ["i2_mesh.hip", "i2_evaluator.hip", "i2_batch.hip", "i2_adjoint.hip", "i2_meshgen.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

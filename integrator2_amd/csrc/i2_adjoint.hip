@@ -1,0 +1,668 @@
+// i2_adjoint.hip -- hand-written adjoint: dL/d(vertex coordinates) of the
+// scalar loss
+//
+//   L = sum over all ordered pairs (i, j) of  w_ij Theta_ij + v_ij . J_ij
+//
+// with Theta_ij the Theta channel of the pair's reported integrals, J_ij its
+// result, and w_ij ~ N(0,1), v_ij ~ N(0,1)^3 drawn counter-based from (seed,
+// pair id i*nT + j), so that no weight is stored.  The refinement tree of
+// every pair (its reported level, from the forward pass's convergence
+// iteration and K), the 2 pi p wrap of vertex-sharing pairs and every branch
+// of the formulas are those of the forward pass (piecewise-constant choices
+// are not differentiated).  Nothing of the forward pass is stored beyond the
+// per-pair convergence iteration: every quadrature point is recomputed.
+//
+//   k_adj_disjoint  disjoint pairs at reported levels 0 and 1.  An item is a
+//                   block of 256 control panels i (one per lane) and a tile of
+//                   panels j that the workgroup walks in step, so K_j is
+//                   uniform: the cotangents of K_i stay in registers for the
+//                   whole tile and those of K_j are summed over each
+//                   wavefront before one set of atomics per wavefront and j.
+//   k_adj_deep      disjoint pairs at reported levels >= 2 (queued by the
+//                   sweep): one wavefront per pair over its 13 * 4^L points.
+//   k_adj_singular  vertex- and edge-sharing pairs: one wavefront per pair;
+//                   reverse mode over the points, then forward-mode dual
+//                   numbers along the 18 vertex coordinates of the pair for
+//                   the per-pair geometry, singular-part contexts and the
+//                   closed-form singular integrals.
+#include "i2_adjoint.hpp"
+#include "i2_adjoint_point.hpp"
+#include "i2_dual.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace i2 {
+using namespace i2d;
+
+// ------------------------------------------------------------ loss weights
+I2_HD __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// N(0,1) by Box-Muller from two 53-bit uniforms in (0, 1) of the splitmix64
+// hash of (seed, pair id, stream); streams 0 (w) and 1..3 (v)
+I2_HD __forceinline__ double normal_draw(uint64_t seed, uint64_t pid, uint64_t stream) {
+    const uint64_t base = mix64(seed ^ mix64(pid * 4ull + stream));
+    const double u1 = ((double)(mix64(base ^ 0x1ull) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = ((double)(mix64(base ^ 0x2ull) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(TWO_PI * u2);
+}
+
+struct Weights { double w; d3 v; };
+__device__ __forceinline__ Weights weights(uint64_t seed, int i, int j, int nT) {
+    const uint64_t pid = (uint64_t)i * (uint64_t)nT + (uint64_t)j;
+    return {normal_draw(seed, pid, 0), {normal_draw(seed, pid, 1), normal_draw(seed, pid, 2), normal_draw(seed, pid, 3)}};
+}
+// cotangent of a pair's (Psi, Theta): dL/dPsi = n_j x v / (4 pi), dL/dTheta = w + v . n_j / (4 pi)
+__device__ __forceinline__ q4 task_cotangent(const Weights& W, d3 nJ) {
+    const d3 a = RECIP_4PI * cross(nJ, W.v);
+    return {a.x, a.y, a.z, W.w + RECIP_4PI * dot(W.v, nJ)};
+}
+
+// ------------------------------------------------------------ helpers
+__device__ __forceinline__ int64_t uniform64a(int64_t x) {
+    const int lo = __builtin_amdgcn_readfirstlane((int)(x & 0xffffffff));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ int find_base_a(const int64_t* base, int n, int64_t x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int64_t rank_disjoint(const AdjMesh& M, int i, int j) {
+    int64_t below = 0;
+    for (int64_t q = M.nbr_start[i]; q < M.nbr_start[i + 1]; ++q) {
+        if (M.nbr[q] < j) ++below; else break;
+    }
+    return M.row_start[i] + (j - i - 1) - below;
+}
+__device__ __forceinline__ int64_t task_disjoint(const AdjMesh& M, int i, int j) {
+    return (i < j) ? rank_disjoint(M, i, j) : M.npairs[I2_NOT_NEIGHBORS] + rank_disjoint(M, j, i);
+}
+__device__ __forceinline__ int reported_level(const AdjArgs& A, int c, int K) {
+    if (A.fixed) return A.last_level;
+    if (A.quirk && c >= 1 && ((K - c) & 1)) return c - 1;
+    return c;
+}
+__device__ __forceinline__ double wsum(double x) {
+    for (int m = 1; m < 64; m <<= 1) x += __shfl_xor(x, m);
+    return x;
+}
+__device__ __forceinline__ void wsum3(d3& v) { v.x = wsum(v.x); v.y = wsum(v.y); v.z = wsum(v.z); }
+__device__ __forceinline__ void wsum4(q4& v) { v.x = wsum(v.x); v.y = wsum(v.y); v.z = wsum(v.z); v.w = wsum(v.w); }
+__device__ __forceinline__ void add3(double* dst, d3 v) {
+    atomicAdd(dst, v.x); atomicAdd(dst + 1, v.y); atomicAdd(dst + 2, v.z);
+}
+__device__ __forceinline__ double dot4(q4 a, q4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+
+struct RuleSh { double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS]; };
+__device__ __forceinline__ void load_rule_sh(RuleSh& s, const Rule& R) {
+    for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
+        s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
+    }
+}
+
+// Vertex cotangents of a panel from those of its tangents and normal
+// (ta = unit(C - B), tb = unit(A - C), tc = unit(B - A), n = unit((B - A) x (C - A)))
+// and of its area (S = |(B - A) x (C - A)| / 2).
+__device__ __forceinline__ void panel_chain(const Panel& P, d3 tab, d3 tbb, d3 tcb, d3 nb, double Sb, d3& Ab, d3& Bb, d3& Cb) {
+    const d3 ua = unit_rev(P.c - P.b, tab);
+    Cb = Cb + ua; Bb = Bb - ua;
+    const d3 ub = unit_rev(P.a - P.c, tbb);
+    Ab = Ab + ub; Cb = Cb - ub;
+    const d3 uc = unit_rev(P.b - P.a, tcb);
+    Bb = Bb + uc; Ab = Ab - uc;
+    const d3 u = P.b - P.a, v = P.c - P.a;
+    const d3 cr = cross(u, v);
+    d3 crb = unit_rev(cr, nb);
+    axpy(crb, 0.5 * Sb, P.n);
+    const d3 gu = cross(v, crb), gv = cross(crb, u);
+    Bb = Bb + gu; Cb = Cb + gv; Ab = Ab - gu - gv;
+}
+
+__device__ __forceinline__ void push_adj_deep(const AdjArgs& A, int m, int i, int j, int level) {
+    const unsigned long long slot = atomicAdd(&A.counters[1], 1ull);
+    if (slot >= (unsigned long long)A.deep_cap) return;   // counted: the host grows the queue and runs again
+    AdjEntry e;
+    e.m = m; e.i = i; e.j = j; e.level = level;
+    A.deep[slot] = e;
+}
+
+// Barycentric weights (w.r.t. the original panel) of the vertices of level-1 child `ch`:
+// children {m_AB, B, m_BC}, {m_BC, C, m_CA}, {m_CA, A, m_AB}, {m_BC, m_CA, m_AB}
+__device__ __forceinline__ void child_bary(int ch, d3& ba, d3& bb, d3& bc) {
+    const d3 A = {1.0, 0.0, 0.0}, B = {0.0, 1.0, 0.0}, C = {0.0, 0.0, 1.0};
+    const d3 mAB = {0.5, 0.5, 0.0}, mBC = {0.0, 0.5, 0.5}, mCA = {0.5, 0.0, 0.5};
+    if (ch == 0) { ba = mAB; bb = B; bc = mBC; }
+    else if (ch == 1) { ba = mBC; bb = C; bc = mCA; }
+    else if (ch == 2) { ba = mCA; bb = A; bc = mAB; }
+    else { ba = mBC; bb = mCA; bc = mAB; }
+}
+
+// ------------------------------------------------------------ disjoint sweep
+constexpr int ADJ_BLOCK = 256;
+
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adj_disjoint(AdjArgs A) {
+    __shared__ RuleSh R;
+    __shared__ long long item_sh;
+    load_rule_sh(R, A.rule);
+    const int lane = threadIdx.x & 63;
+    const int npts = A.rule.n;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_sh = (long long)atomicAdd(&A.counters[0], 1ull);
+        __syncthreads();
+        const int64_t item = uniform64a(item_sh);
+        if (item >= A.disjoint_items) break;
+        const int m = __builtin_amdgcn_readfirstlane(find_base_a(A.item_base, A.nmeshes, item));
+        const AdjMesh& M = A.meshes[m];
+        const int64_t rr = item - A.item_base[m];
+        const int JT = A.j_tile;
+        const int jtiles = (M.nT + JT - 1) / JT;
+        const int iblk = (int)(rr / jtiles), jt = (int)(rr % jtiles);
+        const int i = iblk * ADJ_BLOCK + threadIdx.x;
+        const bool ivalid = i < M.nT;
+        const Panel I = M.panels[ivalid ? i : 0];
+        const int K = M.K[I2_NOT_NEIGHBORS];
+        d3 X0 = {0.0, 0.0, 0.0}, X1 = X0, X2 = X0;   // cotangents of K_i's vertices
+        double Sb = 0.0, loss = 0.0;
+        const int j0 = jt * JT, j1 = min(M.nT, j0 + JT);
+        for (int j = j0; j < j1; ++j) {
+            const Panel J = M.panels[j];
+            bool valid = ivalid && j != i && shared_count(I, J) == 0;
+            int lev = 0;
+            if (valid) {
+                const int64_t z = task_disjoint(M, i, j);
+                lev = reported_level(A, M.conv[I2_NOT_NEIGHBORS][z], K);
+                if (lev >= 2) { push_adj_deep(A, m, i, j, lev); valid = false; }
+            }
+            if (__ballot(valid) == 0ull) continue;
+            Weights W = {0.0, {0.0, 0.0, 0.0}};
+            if (valid) W = weights(A.seed, i, j, M.nT);
+            const q4 gb = task_cotangent(W, J.n);
+            const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+            TPBar g = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            q4 F = q4zero();
+            const int nsub = lev == 1 ? 4 : 1;
+            const double fa = lev == 1 ? 0.25 * I.area : I.area;
+            for (int sub = 0; sub < 4; ++sub) {
+                const bool act = valid && sub < nsub;
+                if (__ballot(act) == 0ull) break;
+                if (!act) continue;
+                d3 a = I.a, b = I.b, c = I.c;
+                d3 ba = {1.0, 0.0, 0.0}, bb = {0.0, 1.0, 0.0}, bc = {0.0, 0.0, 1.0};
+                if (lev == 1) {
+                    const d3 mBC = 0.5 * (I.b + I.c), mCA = 0.5 * (I.c + I.a), mAB = 0.5 * (I.a + I.b);
+                    if (sub == 0) { a = mAB; b = I.b; c = mBC; }
+                    else if (sub == 1) { a = mBC; b = I.c; c = mCA; }
+                    else if (sub == 2) { a = mCA; b = I.a; c = mAB; }
+                    else { a = mBC; b = mCA; c = mAB; }
+                    child_bary(sub, ba, bb, bc);
+                }
+                d3 ab = {0.0, 0.0, 0.0}, bbar = ab, cb = ab;   // cotangents of the sub-panel's vertices
+                for (int k = 0; k < npts; ++k) {
+                    d3 p = R.L1[k] * a;
+                    p = p + R.L2[k] * b;
+                    p = p + R.L3[k] * c;
+                    d3 pb;
+                    const q4 f = theta_psi_fast_rev(p, JG, (fa * R.w[k]) * gb, &pb, g);
+                    F = F + R.w[k] * f;
+                    axpy(ab, R.L1[k], pb); axpy(bbar, R.L2[k], pb); axpy(cb, R.L3[k], pb);
+                }
+                // sub-panel vertices are fixed combinations of K_i's vertices
+                axpy(X0, ba.x, ab); axpy(X0, bb.x, bbar); axpy(X0, bc.x, cb);
+                axpy(X1, ba.y, ab); axpy(X1, bb.y, bbar); axpy(X1, bc.y, cb);
+                axpy(X2, ba.z, ab); axpy(X2, bb.z, bbar); axpy(X2, bc.z, cb);
+            }
+            d3 nb = {0.0, 0.0, 0.0};
+            if (valid) {
+                const q4 Iv = fa * F;
+                Sb += dot4(gb, F) * (fa / I.area);
+                const d3 psi = {Iv.x, Iv.y, Iv.z};
+                nb = RECIP_4PI * (Iv.w * W.v + cross(W.v, psi));
+                const d3 Jr = finalize_plain(Iv, J.n);
+                loss += W.w * Iv.w + dot(W.v, Jr);
+            }
+            // K_j: sum over the wavefront, one set of atomics per wavefront
+            wsum3(g.a); wsum3(g.b); wsum3(g.c); wsum3(g.ta); wsum3(g.tb); wsum3(g.tc); wsum3(nb);
+            if (lane == 0) {
+                d3 Ab = g.a, Bb = g.b, Cb = g.c;
+                panel_chain(J, g.ta, g.tb, g.tc, nb, 0.0, Ab, Bb, Cb);
+                add3(M.dX + 3 * (int64_t)J.v0, Ab);
+                add3(M.dX + 3 * (int64_t)J.v1, Bb);
+                add3(M.dX + 3 * (int64_t)J.v2, Cb);
+            }
+        }
+        if (ivalid && (Sb != 0.0 || dot(X0, X0) + dot(X1, X1) + dot(X2, X2) != 0.0)) {
+            const d3 zero = {0.0, 0.0, 0.0};
+            panel_chain(I, zero, zero, zero, zero, Sb, X0, X1, X2);
+            add3(M.dX + 3 * (int64_t)I.v0, X0);
+            add3(M.dX + 3 * (int64_t)I.v1, X1);
+            add3(M.dX + 3 * (int64_t)I.v2, X2);
+        }
+        loss = wsum(loss);
+        if (lane == 0 && loss != 0.0) atomicAdd(M.loss, loss);
+    }
+}
+
+// ------------------------------------------------------------ deep disjoint pairs
+// One wavefront per queued pair (reported level >= 2) over its npts * 4^L points.
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adj_deep(AdjArgs A) {
+    __shared__ RuleSh R;
+    load_rule_sh(R, A.rule);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int npts = A.rule.n;
+    const unsigned long long cnt = min(A.counters[1], (unsigned long long)A.deep_cap);
+    const int waves = ADJ_BLOCK / 64;
+    for (unsigned long long t = (unsigned long long)blockIdx.x * waves + threadIdx.x / 64; t < cnt;
+         t += (unsigned long long)gridDim.x * waves) {
+        const AdjEntry e = A.deep[t];
+        const AdjMesh& M = A.meshes[e.m];
+        const Panel I = M.panels[e.i];
+        const Panel J = M.panels[e.j];
+        const Weights W = weights(A.seed, e.i, e.j, M.nT);
+        const q4 gb = task_cotangent(W, J.n);
+        const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+        double fa = I.area;
+        for (int l = 0; l < e.level; ++l) fa *= 0.25;
+        TPBar g = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        d3 X0 = {0.0, 0.0, 0.0}, X1 = X0, X2 = X0;
+        q4 F = q4zero();
+        const int64_t total = (int64_t)npts << (2 * e.level);
+        for (int64_t q = lane; q < total; q += 64) {
+            const uint32_t sub = (uint32_t)(q / npts);
+            const int k = (int)(q % npts);
+            d3 a = I.a, b = I.b, c = I.c;
+            sub_panel(a, b, c, e.level, sub);
+            d3 ba = {1.0, 0.0, 0.0}, bb = {0.0, 1.0, 0.0}, bc = {0.0, 0.0, 1.0};
+            sub_panel(ba, bb, bc, e.level, sub);
+            d3 p = R.L1[k] * a;
+            p = p + R.L2[k] * b;
+            p = p + R.L3[k] * c;
+            d3 pb;
+            const q4 f = theta_psi_fast_rev(p, JG, (fa * R.w[k]) * gb, &pb, g);
+            F = F + R.w[k] * f;
+            const d3 beta = R.L1[k] * ba + R.L2[k] * bb + R.L3[k] * bc;
+            axpy(X0, beta.x, pb); axpy(X1, beta.y, pb); axpy(X2, beta.z, pb);
+        }
+        wsum3(g.a); wsum3(g.b); wsum3(g.c); wsum3(g.ta); wsum3(g.tb); wsum3(g.tc);
+        wsum3(X0); wsum3(X1); wsum3(X2); wsum4(F);
+        if (lane == 0) {
+            const q4 Iv = fa * F;
+            const double Sb = dot4(gb, F) * (fa / I.area);
+            const d3 psi = {Iv.x, Iv.y, Iv.z};
+            const d3 nb = RECIP_4PI * (Iv.w * W.v + cross(W.v, psi));
+            const d3 Jr = finalize_plain(Iv, J.n);
+            atomicAdd(M.loss, W.w * Iv.w + dot(W.v, Jr));
+            d3 Ab = g.a, Bb = g.b, Cb = g.c;
+            panel_chain(J, g.ta, g.tb, g.tc, nb, 0.0, Ab, Bb, Cb);
+            add3(M.dX + 3 * (int64_t)J.v0, Ab);
+            add3(M.dX + 3 * (int64_t)J.v1, Bb);
+            add3(M.dX + 3 * (int64_t)J.v2, Cb);
+            const d3 zero = {0.0, 0.0, 0.0};
+            panel_chain(I, zero, zero, zero, zero, Sb, X0, X1, X2);
+            add3(M.dX + 3 * (int64_t)I.v0, X0);
+            add3(M.dX + 3 * (int64_t)I.v1, X1);
+            add3(M.dX + 3 * (int64_t)I.v2, X2);
+        }
+    }
+}
+
+// ------------------------------------------------------------ singular pairs
+using i2ad::D3;
+using i2ad::Dual;
+using i2ad::Q4;
+
+__device__ __forceinline__ D3 dual_vertex(d3 x, int slot, int my_slot, int axis) {
+    D3 r = {Dual(x.x), Dual(x.y), Dual(x.z)};
+    if (slot == my_slot) {
+        if (axis == 0) r.x.d = 1.0; else if (axis == 1) r.y.d = 1.0; else r.z.d = 1.0;
+    }
+    return r;
+}
+__device__ __forceinline__ double dd(d3 bar, D3 x) { return bar.x * x.x.d + bar.y * x.y.d + bar.z * x.z.d; }
+
+struct SingBars {
+    TPBar g;          // thetaPsi parameters (K_j vertices, tangents)
+    EdgeBar ge;       // edge-sharing context
+    VertexBar gv;     // vertex-sharing context
+    d3 X[3];          // K_i vertices (through the quadrature points)
+    q4 F;             // sum_k w_k f
+};
+
+template <int TYPE>
+__device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R, const AdjMesh& M, int i, int j, int level,
+                                              int lane) {
+    const Panel I = M.panels[i];
+    const Panel J = M.panels[j];
+    const Weights W = weights(A.seed, i, j, M.nT);
+    const q4 gb = task_cotangent(W, J.n);
+    const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+    int s1, s2;
+    EdgeCtx ec;
+    VertexCtx vc;
+    if (TYPE == I2_ATTACHED_NEIGHBORS) {
+        shifts_edge(I, J, &s1, &s2);
+        ec = edge_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3));
+    } else {
+        shifts_vertex(I, J, &s1, &s2);
+        vc = vertex_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    }
+    double fa = I.area;
+    for (int l = 0; l < level; ++l) fa *= 0.25;
+    const d3 z3 = {0.0, 0.0, 0.0};
+    SingBars B;
+    B.g = {z3, z3, z3, z3, z3, z3};
+    B.ge = {z3, z3, z3, z3, z3, 0.0};
+    B.gv = {z3, z3, z3, z3, 0.0};
+    B.X[0] = B.X[1] = B.X[2] = z3;
+    B.F = q4zero();
+    const int npts = A.rule.n;
+    const int64_t total = (int64_t)npts << (2 * level);
+    for (int64_t q = lane; q < total; q += 64) {
+        const uint32_t sub = (uint32_t)(q / npts);
+        const int k = (int)(q % npts);
+        d3 a = I.a, b = I.b, c = I.c;
+        sub_panel(a, b, c, level, sub);
+        d3 ba = {1.0, 0.0, 0.0}, bb = {0.0, 1.0, 0.0}, bc = {0.0, 0.0, 1.0};
+        sub_panel(ba, bb, bc, level, sub);
+        d3 p = R.L1[k] * a;
+        p = p + R.L2[k] * b;
+        p = p + R.L3[k] * c;
+        const q4 gp = (fa * R.w[k]) * gb;
+        const q4 gm = (-1.0) * gp;
+        d3 pb, ps;
+        q4 f = theta_psi_fast_rev(p, JG, gp, &pb, B.g);
+        if (TYPE == I2_ATTACHED_NEIGHBORS) f = f - sing_edge_rev(p, ec, gm, &ps, B.ge);
+        else f = f - sing_vertex_rev(p, vc, gm, &ps, B.gv);
+        pb = pb + ps;
+        B.F = B.F + R.w[k] * f;
+        const d3 beta = R.L1[k] * ba + R.L2[k] * bb + R.L3[k] * bc;
+        axpy(B.X[0], beta.x, pb); axpy(B.X[1], beta.y, pb); axpy(B.X[2], beta.z, pb);
+    }
+    wsum3(B.g.a); wsum3(B.g.b); wsum3(B.g.c); wsum3(B.g.ta); wsum3(B.g.tb); wsum3(B.g.tc);
+    wsum3(B.X[0]); wsum3(B.X[1]); wsum3(B.X[2]); wsum4(B.F);
+    if (TYPE == I2_ATTACHED_NEIGHBORS) {
+        wsum3(B.ge.JB); wsum3(B.ge.JC); wsum3(B.ge.taua); wsum3(B.ge.taub); wsum3(B.ge.tauc); B.ge.ilvc = wsum(B.ge.ilvc);
+    } else {
+        wsum3(B.gv.JA); wsum3(B.gv.taua); wsum3(B.gv.taub); wsum3(B.gv.e); B.gv.invAri = wsum(B.gv.invAri);
+    }
+    // whole-pair values (uniform over the wavefront)
+    q4 sing;
+    if (TYPE == I2_ATTACHED_NEIGHBORS)
+        sing = integrate_sing_edge(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                   vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    else
+        sing = integrate_sing_vertex(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                     vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    const q4 Iv = fa * B.F + sing;
+    const int pw = TYPE == I2_SIMPLE_NEIGHBORS ? wrap_count(Iv.w, I.area) : 0;
+    const d3 Jr = TYPE == I2_SIMPLE_NEIGHBORS ? finalize_vertex_p(Iv, J.n, I.area, pw) : finalize_plain(Iv, J.n);
+    if (lane == 0) atomicAdd(M.loss, W.w * Iv.w + dot(W.v, Jr));
+    const d3 psi = {Iv.x, Iv.y, Iv.z};
+    const double thetaw = Iv.w + 2.0 * pw * (TWO_PI * I.area);
+    const d3 nb = RECIP_4PI * (thetaw * W.v + cross(W.v, psi));
+    const double Sb = dot4(gb, B.F) * (fa / I.area) + (TYPE == I2_SIMPLE_NEIGHBORS ? pw * dot(W.v, J.n) : 0.0);
+    // forward mode along coordinate d = 3 * slot + axis, slots 0..2 = K_i's, 3..5 = K_j's vertices
+    if (lane >= 18) return;
+    const int slot = lane / 3, axis = lane % 3;
+    const D3 ia = dual_vertex(I.a, 0, slot, axis), ib = dual_vertex(I.b, 1, slot, axis), ic = dual_vertex(I.c, 2, slot, axis);
+    const D3 ja = dual_vertex(J.a, 3, slot, axis), jb = dual_vertex(J.b, 4, slot, axis), jc = dual_vertex(J.c, 5, slot, axis);
+    const D3 cri = cross(ib - ia, ic - ia), crj = cross(jb - ja, jc - ja);
+    const D3 nI = unit(cri), nJ = unit(crj);
+    const Dual SI = vlen(cri) * Dual(0.5);
+    const D3 ta = unit(jc - jb), tb = unit(ja - jc), tc = unit(jb - ja);
+    const D3 vi[3] = {ia, ib, ic}, vj[3] = {ja, jb, jc};
+    double dl = 0.0;
+    if (slot < 3) dl += (axis == 0 ? B.X[slot].x : (axis == 1 ? B.X[slot].y : B.X[slot].z));
+    else {
+        const d3 gd = slot == 3 ? B.g.a : (slot == 4 ? B.g.b : B.g.c);
+        dl += (axis == 0 ? gd.x : (axis == 1 ? gd.y : gd.z));
+    }
+    dl += dd(B.g.ta, ta) + dd(B.g.tb, tb) + dd(B.g.tc, tc) + dd(nb, nJ) + Sb * SI.d;
+    Q4 sd;
+    if (TYPE == I2_ATTACHED_NEIGHBORS) {
+        const EdgeCtxT<Dual> c = edge_ctx(vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3]);
+        dl += dd(B.ge.JB, c.JB) + dd(B.ge.JC, c.JC) + dd(B.ge.taua, c.taua) + dd(B.ge.taub, c.taub) +
+              dd(B.ge.tauc, c.tauc) + B.ge.ilvc * c.ilvc.d;
+        sd = integrate_sing_edge(vi[s1], vi[(s1 + 1) % 3], vi[(s1 + 2) % 3], vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3],
+                                 nI, nJ, SI);
+    } else {
+        const VertexCtxT<Dual> c = vertex_ctx(vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3], nI, nJ, SI);
+        dl += dd(B.gv.JA, c.JA) + dd(B.gv.taua, c.taua) + dd(B.gv.taub, c.taub) + dd(B.gv.e, c.e) + B.gv.invAri * c.invAri.d;
+        sd = integrate_sing_vertex(vi[s1], vi[(s1 + 1) % 3], vi[(s1 + 2) % 3], vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3],
+                                   nI, nJ, SI);
+    }
+    dl += gb.x * sd.x.d + gb.y * sd.y.d + gb.z * sd.z.d + gb.w * sd.w.d;
+    const int vid = slot == 0 ? I.v0 : slot == 1 ? I.v1 : slot == 2 ? I.v2 : slot == 3 ? J.v0 : slot == 4 ? J.v1 : J.v2;
+    atomicAdd(M.dX + 3 * (int64_t)vid + axis, dl);
+}
+
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adj_singular(AdjArgs A) {
+    __shared__ RuleSh R;
+    load_rule_sh(R, A.rule);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int waves = ADJ_BLOCK / 64;
+    for (int64_t t = (int64_t)blockIdx.x * waves + threadIdx.x / 64; t < A.singular_tasks; t += (int64_t)gridDim.x * waves) {
+        const int b = find_base_a(A.sing_base, 2 * A.nmeshes, t);
+        const int type = b / A.nmeshes == 0 ? I2_SIMPLE_NEIGHBORS : I2_ATTACHED_NEIGHBORS;
+        const AdjMesh& M = A.meshes[b % A.nmeshes];
+        const int64_t z = t - A.sing_base[b];
+        const int64_t n = M.npairs[type];
+        const int32_t* L = M.lists[type];
+        const int64_t k = z < n ? z : z - n;
+        int i = L[3 * k], j = L[3 * k + 1];
+        if (z >= n) { const int s = i; i = j; j = s; }
+        const int level = reported_level(A, M.conv[type][z], M.K[type]);
+        if (type == I2_ATTACHED_NEIGHBORS) singular_task<I2_ATTACHED_NEIGHBORS>(A, R, M, i, j, level, lane);
+        else singular_task<I2_SIMPLE_NEIGHBORS>(A, R, M, i, j, level, lane);
+    }
+}
+
+__global__ void k_adj_zero(double* p, int64_t n, unsigned long long* c, int nc) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) p[q] = 0.0;
+    if (blockIdx.x == 0 && (int)threadIdx.x < nc) c[threadIdx.x] = 0ull;
+}
+
+// ------------------------------------------------------------ host side
+int adjoint_launch(AdjState& S, hipStream_t st) {
+    const int64_t nzero = (int64_t)S.d_grad.n;
+    hipLaunchKernelGGL(k_adj_zero, dim3(256), dim3(256), 0, st, S.d_grad.p, nzero, S.d_counters.p, 4);
+    I2_TRY(check_launch("k_adj_zero"));
+    hipLaunchKernelGGL(k_adj_disjoint, dim3(S.grid), dim3(ADJ_BLOCK), 0, st, S.args);
+    I2_TRY(check_launch("k_adj_disjoint"));
+    hipLaunchKernelGGL(k_adj_deep, dim3(S.grid), dim3(ADJ_BLOCK), 0, st, S.args);
+    I2_TRY(check_launch("k_adj_deep"));
+    if (S.args.singular_tasks > 0) {
+        hipLaunchKernelGGL(k_adj_singular, dim3(S.grid), dim3(ADJ_BLOCK), 0, st, S.args);
+        I2_TRY(check_launch("k_adj_singular"));
+    }
+    return I2_OK;
+}
+
+// Builds the descriptors of one mesh's adjoint from the forward pass's
+// convergence iterations and K per type.
+int adjoint_setup(AdjState& S, int device, const i2_mesh* mesh, const i2_integrator* it, const int8_t* const conv[3],
+                  const int K[3], uint64_t seed) {
+    int cus = 0;
+    I2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    S.grid = std::max(1, cus) * 4;
+    AdjMesh D{};
+    D.panels = mesh->d_panels.p;
+    D.row_start = mesh->d_row_start.p;
+    D.nbr_start = mesh->d_nbr_start.p;
+    D.nbr = mesh->d_nbr.p;
+    D.lists[0] = mesh->d_simple.p;
+    D.lists[1] = mesh->d_attached.p;
+    for (int t = 0; t < 3; ++t) { D.npairs[t] = mesh->pairs[t]; D.conv[t] = conv[t]; }
+    D.nT = (int)mesh->nT;
+    const int64_t ngrad = 3 * mesh->nV + 1;     // dL/dX, then L
+    I2_TRY(S.d_grad.ensure(ngrad));
+    D.dX = S.d_grad.p;
+    D.loss = S.d_grad.p + 3 * mesh->nV;
+    S.nV = mesh->nV;
+    I2_TRY(S.d_K.ensure(3));
+    I2_HIP(hipMemcpy(S.d_K.p, K, 3 * sizeof(int), hipMemcpyHostToDevice));
+    D.K = S.d_K.p;
+    I2_TRY(S.d_mesh.ensure(1));
+    I2_HIP(hipMemcpy(S.d_mesh.p, &D, sizeof(AdjMesh), hipMemcpyHostToDevice));
+    AdjArgs& A = S.args;
+    A.meshes = S.d_mesh.p;
+    A.nmeshes = 1;
+    A.fixed = it->fixed;
+    A.last_level = it->fixed ? it->fixed_level : it->max_level;
+    A.quirk = it->quirk;
+    A.rule = it->rule;
+    A.seed = seed;
+    A.j_tile = 32;
+    const int64_t iblocks = (mesh->nT + ADJ_BLOCK - 1) / ADJ_BLOCK;
+    const int64_t jtiles = (mesh->nT + A.j_tile - 1) / A.j_tile;
+    const int64_t h_item_base[2] = {0, iblocks * jtiles};
+    A.disjoint_items = iblocks * jtiles;
+    I2_TRY(S.d_item_base.ensure(2));
+    I2_HIP(hipMemcpy(S.d_item_base.p, h_item_base, sizeof(h_item_base), hipMemcpyHostToDevice));
+    A.item_base = S.d_item_base.p;
+    const int64_t h_sing_base[3] = {0, 2 * mesh->pairs[0], 2 * mesh->pairs[0] + 2 * mesh->pairs[1]};
+    A.singular_tasks = h_sing_base[2];
+    I2_TRY(S.d_sing_base.ensure(3));
+    I2_HIP(hipMemcpy(S.d_sing_base.p, h_sing_base, sizeof(h_sing_base), hipMemcpyHostToDevice));
+    A.sing_base = S.d_sing_base.p;
+    I2_TRY(S.d_counters.ensure(4));
+    A.counters = S.d_counters.p;
+    if (S.deep_cap == 0) S.deep_cap = 1 << 16;
+    I2_TRY(S.d_deep.ensure(S.deep_cap));
+    A.deep = S.d_deep.p;
+    A.deep_cap = S.deep_cap;
+    return I2_OK;
+}
+
+// Runs the adjoint, growing the deep queue after an overflow.
+int adjoint_run(AdjState& S, hipStream_t st, double* loss, double* dLdX) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        I2_HIP(hipEventRecord(S.e0, st));
+        I2_TRY(adjoint_launch(S, st));
+        I2_HIP(hipEventRecord(S.e1, st));
+        unsigned long long cnt = 0;
+        I2_HIP(hipMemcpyAsync(&cnt, S.d_counters.p + 1, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        I2_HIP(hipStreamSynchronize(st));
+        if (cnt <= (unsigned long long)S.deep_cap) {
+            float ms = 0.0f;
+            I2_HIP(hipEventElapsedTime(&ms, S.e0, S.e1));
+            S.ms = ms;
+            if (dLdX) I2_HIP(hipMemcpy(dLdX, S.d_grad.p, sizeof(double) * 3 * S.nV, hipMemcpyDeviceToHost));
+            if (loss) I2_HIP(hipMemcpy(loss, S.d_grad.p + 3 * S.nV, sizeof(double), hipMemcpyDeviceToHost));
+            return I2_OK;
+        }
+        S.deep_cap = (int64_t)(cnt + cnt / 4 + 1);
+        I2_TRY(S.d_deep.ensure(S.deep_cap));
+        S.args.deep = S.d_deep.p;
+        S.args.deep_cap = S.deep_cap;
+    }
+    set_error("adjoint deep-pair queue did not converge");
+    return I2_ERR_STATE;
+}
+
+void adjoint_release(AdjState*& S) {
+    if (!S) return;
+    if (S->e0) (void)hipEventDestroy(S->e0);
+    if (S->e1) (void)hipEventDestroy(S->e1);
+    delete S;
+    S = nullptr;
+}
+
+}  // namespace i2
+
+using namespace i2;
+
+extern "C" {
+
+int i2_evaluator_loss_and_grad(i2_evaluator* ev, uint64_t seed, double* loss, double* dLdX) {
+    if (!ev || !loss || !dLdX) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!ev->ran || !ev->all_pairs) { set_error("run all pairs first: the loss is defined over every ordered pair"); return I2_ERR_STATE; }
+    DeviceGuard g(ev->mesh->device);
+    if (!g.ok) return I2_ERR_HIP;
+    if (!ev->adj) {
+        ev->adj = new AdjState;
+        if (hipEventCreate(&ev->adj->e0) != hipSuccess || hipEventCreate(&ev->adj->e1) != hipSuccess) {
+            (void)hipGetLastError();
+            adjoint_release(ev->adj);
+            set_error("event creation failed");
+            return I2_ERR_HIP;
+        }
+    }
+    const int8_t* conv[3];
+    int K[3];
+    for (int t = 0; t < 3; ++t) {
+        K[t] = ev->type[t].K;
+        if (ev->path == 1) {
+            const double4* ints;
+            const double* res;
+            engine_evaluator_outputs(ev, t, &ints, &res, &conv[t]);
+        } else {
+            conv[t] = ev->type[t].conv.p;
+        }
+    }
+    I2_TRY(adjoint_setup(*ev->adj, ev->mesh->device, ev->mesh, ev->integ, conv, K, seed));
+    return adjoint_run(*ev->adj, ev->stream, loss, dLdX);
+}
+
+int i2_evaluator_adjoint_ms(const i2_evaluator* ev, double* ms) {
+    if (!ev || !ms) { set_error("null argument"); return I2_ERR_INVALID; }
+    if (!ev->adj) { set_error("no adjoint run"); return I2_ERR_STATE; }
+    *ms = ev->adj->ms;
+    return I2_OK;
+}
+
+int i2_host_loss_weights(uint64_t seed, const int32_t* ij, int64_t n, int nT, double* out4) {
+    if (!ij || !out4 || n < 0 || nT <= 0) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    for (int64_t q = 0; q < n; ++q) {
+        const uint64_t pid = (uint64_t)ij[2 * q] * (uint64_t)nT + (uint64_t)ij[2 * q + 1];
+        for (int s = 0; s < 4; ++s) out4[4 * q + s] = normal_draw(seed, pid, (uint64_t)s);
+    }
+    return I2_OK;
+}
+
+int i2_host_point_adjoint(int kind, const double* p3, const double* params, const double* gbar4, double* f4, double* pbar3,
+                          double* parbar) {
+    if (!p3 || !params || !gbar4 || !f4 || !pbar3 || !parbar) { set_error("null argument"); return I2_ERR_INVALID; }
+    const d3 p = {p3[0], p3[1], p3[2]};
+    const q4 gb = {gbar4[0], gbar4[1], gbar4[2], gbar4[3]};
+    auto v = [&](int o) { return d3{params[o], params[o + 1], params[o + 2]}; };
+    auto put = [&](int o, d3 x) { parbar[o] = x.x; parbar[o + 1] = x.y; parbar[o + 2] = x.z; };
+    const d3 z3 = {0.0, 0.0, 0.0};
+    d3 pb;
+    q4 f;
+    if (kind == 0) {
+        const JGeom J = {v(0), v(3), v(6), v(9), v(12), v(15)};
+        TPBar g = {z3, z3, z3, z3, z3, z3};
+        f = theta_psi_fast_rev(p, J, gb, &pb, g);
+        put(0, g.a); put(3, g.b); put(6, g.c); put(9, g.ta); put(12, g.tb); put(15, g.tc);
+    } else if (kind == 1) {
+        const EdgeCtx c = {v(0), v(3), v(6), v(9), v(12), params[15]};
+        EdgeBar g = {z3, z3, z3, z3, z3, 0.0};
+        f = sing_edge_rev(p, c, gb, &pb, g);
+        put(0, g.JB); put(3, g.JC); put(6, g.taua); put(9, g.taub); put(12, g.tauc); parbar[15] = g.ilvc;
+    } else if (kind == 2) {
+        const VertexCtx c = {v(0), v(3), v(6), v(9), params[12]};
+        VertexBar g = {z3, z3, z3, z3, 0.0};
+        f = sing_vertex_rev(p, c, gb, &pb, g);
+        put(0, g.JA); put(3, g.taua); put(6, g.taub); put(9, g.e); parbar[12] = g.invAri;
+    } else {
+        set_error("unknown point function %d", kind);
+        return I2_ERR_INVALID;
+    }
+    f4[0] = f.x; f4[1] = f.y; f4[2] = f.z; f4[3] = f.w;
+    pbar3[0] = pb.x; pbar3[1] = pb.y; pbar3[2] = pb.z;
+    return I2_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// i2_adjoint.hpp -- descriptors of the adjoint kernels (dL/d vertex coordinates).
+#pragma once
+#include "i2_batch.hpp"
+
+namespace i2 {
+
+// One mesh as the adjoint kernels see it: the forward pass's outputs they
+// need (convergence iteration per ordered task, K per type) and the
+// accumulators they write.
+struct AdjMesh {
+    const i2d::Panel* panels;
+    const int64_t* row_start;
+    const int64_t* nbr_start;
+    const int32_t* nbr;
+    const int32_t* lists[2];      // vertex- and edge-sharing (i, j, k), i < j
+    int64_t npairs[3];
+    const int8_t* conv[3];        // per ordered task, reference task order
+    const int* K;                 // refinement iterations per type
+    int nT;
+    int pad;
+    double* dX;                   // dL/dX, 3 nV
+    double* loss;                 // L
+};
+
+// A disjoint pair whose reported level is >= 2.
+struct AdjEntry { int m, i, j, level; };
+
+struct AdjArgs {
+    const AdjMesh* meshes;
+    int nmeshes;
+    int fixed;
+    int last_level;
+    int quirk;
+    i2d::Rule rule;
+    uint64_t seed;
+    int j_tile;                   // panels j per disjoint item
+    int pad;
+    int64_t disjoint_items;       // items: (mesh, block of 256 panels i, tile of panels j)
+    const int64_t* item_base;     // per mesh
+    int64_t singular_tasks;       // ordered vertex- and edge-sharing tasks of all meshes
+    const int64_t* sing_base;     // [kind * nmeshes + m], kind 0 vertex-, 1 edge-sharing
+    unsigned long long* counters; // [0] disjoint item, [1] deep pairs queued
+    AdjEntry* deep;
+    int64_t deep_cap;
+};
+
+struct AdjState {
+    AdjArgs args{};
+    int grid = 0;
+    int64_t nV = 0;
+    int64_t deep_cap = 0;
+    double ms = 0.0;              // device time of the last adjoint run
+    DBuf<AdjMesh> d_mesh;
+    DBuf<int> d_K;
+    DBuf<double> d_grad;          // dL/dX (3 nV) and L
+    DBuf<int64_t> d_item_base, d_sing_base;
+    DBuf<unsigned long long> d_counters;
+    DBuf<AdjEntry> d_deep;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+void adjoint_release(AdjState*& S);
+
+}  // namespace i2

@@ -1,0 +1,185 @@
+// i2_adjoint_point.hpp -- reverse mode of the per-point functions of the
+// quadrature loops, derived by hand: the inner integral thetaPsi (the
+// division-free form theta_psi_fast of i2_device.hpp) and the two singular
+// parts (evaluatorJ3DK.cu:315-350, 352-407).
+//
+// Each function recomputes the forward intermediates of one point, returns the
+// forward value f, and adds gb . df/dx to the cotangents of the point p and of
+// the function's parameters (triangle vertices and tangents, or the singular
+// context).  Parameters are treated as free variables; the per-task chain from
+// the parameters to the vertex coordinates is applied once per task.
+#pragma once
+#include "i2_device.hpp"
+
+namespace i2d {
+
+I2_HD __forceinline__ void axpy(d3& y, double a, d3 x) { y.x += a * x.x; y.y += a * x.y; y.z += a * x.z; }
+
+// cotangents of the parameters of thetaPsi over K_j: vertices and edge tangents
+struct TPBar { d3 a, b, c, ta, tb, tc; };
+
+I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, q4 gb, d3* pbar, TPBar& g) {
+    // ---- forward, as theta_psi_fast
+    const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
+    const double la = sqrt(dot(da, da)), lb = sqrt(dot(db, db)), lc = sqrt(dot(dc, dc));
+    double n1 = la + dot(da, J.tc), e1 = lb + dot(db, J.tc);
+    double n2 = lb + dot(db, J.ta), e2 = lc + dot(dc, J.ta);
+    double n3 = lc + dot(dc, J.tb), e3 = la + dot(da, J.tb);
+    const double h = 0.5 * EPS_PSI_THETA2;
+    const bool f1 = fabs(e1) < h * lb, f2 = fabs(e2) < h * lc, f3 = fabs(e3) < h * la;
+    if (f1) { n1 = lb; e1 = la; }
+    if (f2) { n2 = lc; e2 = lb; }
+    if (f3) { n3 = la; e3 = lc; }
+    const double e12 = e1 * e2;
+    const double inv = 1.0 / (e12 * e3);
+    const double t1 = i2m::log_pos(n1 * (e2 * e3) * inv);
+    const double t2 = i2m::log_pos(n2 * (e1 * e3) * inv);
+    const double t3 = i2m::log_pos(n3 * e12 * inv);
+    q4 r;
+    r.x = t1 * J.tc.x + t2 * J.ta.x + t3 * J.tb.x;
+    r.y = t1 * J.tc.y + t2 * J.ta.y + t3 * J.tb.y;
+    r.z = t1 * J.tc.z + t2 * J.ta.z + t3 * J.tb.z;
+    const double y = dot(cross(da, db), dc);
+    const double x = la * lb * lc + dot(da, db) * lc + dot(db, dc) * la + dot(dc, da) * lb;
+    r.w = 2.0 * i2m::atan2_fast(y, x);
+    // ---- reverse
+    const d3 gP = {gb.x, gb.y, gb.z};
+    const double tb1 = dot(gP, J.tc), tb2 = dot(gP, J.ta), tb3 = dot(gP, J.tb);
+    axpy(g.tc, t1, gP); axpy(g.ta, t2, gP); axpy(g.tb, t3, gP);
+    // t_k = log n_k - log e_k: reciprocals from two divisions
+    const double invn = 1.0 / (n1 * n2 * n3);
+    const double rn1 = n2 * n3 * invn, rn2 = n1 * n3 * invn, rn3 = n1 * n2 * invn;
+    const double re1 = e2 * e3 * inv, re2 = e1 * e3 * inv, re3 = e12 * inv;
+    const double nb1 = tb1 * rn1, eb1 = -tb1 * re1;
+    const double nb2 = tb2 * rn2, eb2 = -tb2 * re2;
+    const double nb3 = tb3 * rn3, eb3 = -tb3 * re3;
+    double lab = 0.0, lbb = 0.0, lcb = 0.0;
+    d3 dab = {0.0, 0.0, 0.0}, dbb = dab, dcb = dab;
+    if (!f1) { lab += nb1; axpy(dab, nb1, J.tc); axpy(g.tc, nb1, da); lbb += eb1; axpy(dbb, eb1, J.tc); axpy(g.tc, eb1, db); }
+    else { lbb += nb1; lab += eb1; }
+    if (!f2) { lbb += nb2; axpy(dbb, nb2, J.ta); axpy(g.ta, nb2, db); lcb += eb2; axpy(dcb, eb2, J.ta); axpy(g.ta, eb2, dc); }
+    else { lcb += nb2; lbb += eb2; }
+    if (!f3) { lcb += nb3; axpy(dcb, nb3, J.tb); axpy(g.tb, nb3, dc); lab += eb3; axpy(dab, eb3, J.tb); axpy(g.tb, eb3, da); }
+    else { lab += nb3; lcb += eb3; }
+    // Theta = 2 atan2(y, x)
+    const double r2 = 1.0 / (x * x + y * y);
+    const double yb = 2.0 * gb.w * x * r2, xb = -2.0 * gb.w * y * r2;
+    axpy(dab, yb, cross(db, dc)); axpy(dbb, yb, cross(dc, da)); axpy(dcb, yb, cross(da, db));
+    lab += xb * (lb * lc + dot(db, dc));
+    lbb += xb * (la * lc + dot(dc, da));
+    lcb += xb * (la * lb + dot(da, db));
+    axpy(dab, xb * lc, db); axpy(dab, xb * lb, dc);
+    axpy(dbb, xb * lc, da); axpy(dbb, xb * la, dc);
+    axpy(dcb, xb * la, db); axpy(dcb, xb * lb, da);
+    // l = |d|
+    const double il = 1.0 / (la * lb * lc);
+    axpy(dab, lab * (lb * lc * il), da);
+    axpy(dbb, lbb * (la * lc * il), db);
+    axpy(dcb, lcb * (la * lb * il), dc);
+    *pbar = dab + dbb + dcb;
+    g.a = g.a - dab; g.b = g.b - dbb; g.c = g.c - dcb;
+    return r;
+}
+
+// d(u / |u|) backwards: cotangent of u from that of the unit vector
+I2_HD __forceinline__ d3 unit_rev(d3 u, d3 ub) {
+    const double l = sqrt(dot(u, u));
+    const d3 t = (1.0 / l) * u;
+    return (1.0 / l) * (ub - dot(ub, t) * t);
+}
+
+// cotangents of the edge-sharing context
+struct EdgeBar { d3 JB, JC, taua, taub, tauc; double ilvc; };
+
+I2_HD __forceinline__ q4 sing_edge_rev(d3 p, const EdgeCtx& c, q4 gb, d3* pbar, EdgeBar& g) {
+    // ---- forward, as sing_edge
+    const d3 pa = p - c.JB, pb = p - c.JC;
+    const double lva = vlen(pa), lvb = vlen(pb);
+    const d3 va = divs(pa, lva), vb = divs(pb, lvb);
+    const double P1 = dot(c.tauc, c.tauc - vb), Q1 = dot(c.tauc, c.tauc - va);
+    const double P2 = dot(c.taub, c.taub + va), P3 = dot(c.taua, c.taua - vb);
+    const double l1 = log((lvb * P1) / (lva * Q1));
+    const double l2 = log(lva * P2 * c.ilvc);
+    const double l3 = log(lvb * P3 * c.ilvc);
+    q4 r;
+    r.x = l1 * c.tauc.x - l2 * c.taub.x - l3 * c.taua.x;
+    r.y = l1 * c.tauc.y - l2 * c.taub.y - l3 * c.taua.y;
+    r.z = l1 * c.tauc.z - l2 * c.taub.z - l3 * c.taua.z;
+    const double y1 = dot(cross(va, c.taub), c.tauc), x1 = dot(c.taub - c.tauc, c.taub + va);
+    const double y2 = dot(cross(vb, c.taua), c.tauc), x2 = dot(c.taua - c.tauc, c.taua - vb);
+    r.w = 2.0 * (atan2(y1, x1) - atan2(y2, x2));
+    // ---- reverse
+    const d3 gP = {gb.x, gb.y, gb.z};
+    const double l1b = dot(gP, c.tauc), l2b = -dot(gP, c.taub), l3b = -dot(gP, c.taua);
+    axpy(g.tauc, l1, gP); axpy(g.taub, -l2, gP); axpy(g.taua, -l3, gP);
+    double lvab = -l1b / lva + l2b / lva, lvbb = l1b / lvb + l3b / lvb;
+    const double P1b = l1b / P1, Q1b = -l1b / Q1, P2b = l2b / P2, P3b = l3b / P3;
+    g.ilvc += (l2b + l3b) / c.ilvc;
+    d3 vab = {0.0, 0.0, 0.0}, vbb = vab;
+    axpy(g.tauc, P1b, 2.0 * c.tauc - vb); axpy(vbb, -P1b, c.tauc);
+    axpy(g.tauc, Q1b, 2.0 * c.tauc - va); axpy(vab, -Q1b, c.tauc);
+    axpy(g.taub, P2b, 2.0 * c.taub + va); axpy(vab, P2b, c.taub);
+    axpy(g.taua, P3b, 2.0 * c.taua - vb); axpy(vbb, -P3b, c.taua);
+    const double th1b = 2.0 * gb.w, th2b = -2.0 * gb.w;
+    const double q1 = 1.0 / (x1 * x1 + y1 * y1), q2 = 1.0 / (x2 * x2 + y2 * y2);
+    const double y1b = th1b * x1 * q1, x1b = -th1b * y1 * q1;
+    const double y2b = th2b * x2 * q2, x2b = -th2b * y2 * q2;
+    axpy(vab, y1b, cross(c.taub, c.tauc)); axpy(g.taub, y1b, cross(c.tauc, va)); axpy(g.tauc, y1b, cross(va, c.taub));
+    axpy(g.taub, x1b, (c.taub + va) + (c.taub - c.tauc)); axpy(g.tauc, -x1b, c.taub + va); axpy(vab, x1b, c.taub - c.tauc);
+    axpy(vbb, y2b, cross(c.taua, c.tauc)); axpy(g.taua, y2b, cross(c.tauc, vb)); axpy(g.tauc, y2b, cross(vb, c.taua));
+    axpy(g.taua, x2b, (c.taua - vb) + (c.taua - c.tauc)); axpy(g.tauc, -x2b, c.taua - vb); axpy(vbb, -x2b, c.taua - c.tauc);
+    // va = pa / |pa|, vb = pb / |pb|
+    d3 pab = (1.0 / lva) * (vab - dot(vab, va) * va);
+    d3 pbb = (1.0 / lvb) * (vbb - dot(vbb, vb) * vb);
+    axpy(pab, lvab, va);
+    axpy(pbb, lvbb, vb);
+    *pbar = pab + pbb;
+    g.JB = g.JB - pab;
+    g.JC = g.JC - pbb;
+    return r;
+}
+
+// cotangents of the vertex-sharing context
+struct VertexBar { d3 JA, taua, taub, e; double invAri; };
+
+I2_HD __forceinline__ q4 sing_vertex_rev(d3 p, const VertexCtx& c, q4 gb, d3* pbar, VertexBar& g) {
+    // ---- forward, as sing_vertex
+    const d3 pc = p - c.JA;
+    const double lvc = vlen(pc);
+    const d3 o = divs(pc, lvc);
+    const double A1 = 1.0 + dot(c.taua, o), B1 = 1.0 - dot(c.taub, o);
+    const double la = log((lvc * A1) * c.invAri);
+    const double lb = log((lvc * B1) * c.invAri);
+    q4 r;
+    r.x = -(la * c.taua.x + lb * c.taub.x);
+    r.y = -(la * c.taua.y + lb * c.taub.y);
+    r.z = -(la * c.taua.z + lb * c.taub.z);
+    const double y1 = dot(cross(o, c.taua), c.e), x1 = dot(c.e - o, c.e - c.taua);
+    const double y2 = dot(cross(o, c.taub), c.e), x2 = dot(c.e - o, c.e + c.taub);
+    r.w = 2.0 * (atan2(y1, x1) + atan2(y2, x2));
+    // ---- reverse
+    const d3 gP = {gb.x, gb.y, gb.z};
+    const double lab = -dot(gP, c.taua), lbb = -dot(gP, c.taub);
+    axpy(g.taua, -la, gP); axpy(g.taub, -lb, gP);
+    const double lvcb = (lab + lbb) / lvc;
+    g.invAri += (lab + lbb) / c.invAri;
+    const double A1b = lab / A1, B1b = lbb / B1;
+    d3 ob = {0.0, 0.0, 0.0};
+    axpy(g.taua, A1b, o); axpy(ob, A1b, c.taua);
+    axpy(g.taub, -B1b, o); axpy(ob, -B1b, c.taub);
+    const double thb = 2.0 * gb.w;
+    const double q1 = 1.0 / (x1 * x1 + y1 * y1), q2 = 1.0 / (x2 * x2 + y2 * y2);
+    const double y1b = thb * x1 * q1, x1b = -thb * y1 * q1;
+    const double y2b = thb * x2 * q2, x2b = -thb * y2 * q2;
+    axpy(ob, y1b, cross(c.taua, c.e)); axpy(g.taua, y1b, cross(c.e, o)); axpy(g.e, y1b, cross(o, c.taua));
+    axpy(g.e, x1b, (c.e - c.taua) + (c.e - o)); axpy(ob, -x1b, c.e - c.taua); axpy(g.taua, -x1b, c.e - o);
+    axpy(ob, y2b, cross(c.taub, c.e)); axpy(g.taub, y2b, cross(c.e, o)); axpy(g.e, y2b, cross(o, c.taub));
+    axpy(g.e, x2b, (c.e + c.taub) + (c.e - o)); axpy(ob, -x2b, c.e + c.taub); axpy(g.taub, x2b, c.e - o);
+    d3 pcb = (1.0 / lvc) * (ob - dot(ob, o) * o);
+    axpy(pcb, lvcb, o);
+    *pbar = pcb;
+    g.JA = g.JA - pcb;
+    return r;
+}
+
+}  // namespace i2d

@@ -43,23 +43,32 @@ struct Rule {
 };
 
 // ------------------------------------------------------------ vectors
-struct d3 { double x, y, z; };
-I2_HD __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-I2_HD __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-I2_HD __forceinline__ d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
-I2_HD __forceinline__ d3 operator*(double s, d3 a) { return {a.x * s, a.y * s, a.z * s}; }
-I2_HD __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-I2_HD __forceinline__ d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-I2_HD __forceinline__ double vlen(d3 a) { return sqrt(dot(a, a)); }
-I2_HD __forceinline__ d3 unit(d3 a) { const double inv = 1.0 / vlen(a); return {a.x * inv, a.y * inv, a.z * inv}; }
-I2_HD __forceinline__ d3 divs(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+// Generic over the scalar so that the per-task geometry and closed forms can
+// also run in forward-mode dual numbers (the adjoint); d3 and q4 are the FP64
+// types of the forward kernels.  val() is the value used by branch tests.
+I2_HD __forceinline__ double val(double x) { return x; }
+
+template <class T> struct v3t { T x, y, z; };
+using d3 = v3t<double>;
+template <class T> I2_HD __forceinline__ v3t<T> operator+(v3t<T> a, v3t<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> I2_HD __forceinline__ v3t<T> operator-(v3t<T> a, v3t<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> I2_HD __forceinline__ v3t<T> operator-(v3t<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class S, class T> I2_HD __forceinline__ v3t<T> operator*(S s, v3t<T> a) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> I2_HD __forceinline__ T dot(v3t<T> a, v3t<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> I2_HD __forceinline__ v3t<T> cross(v3t<T> a, v3t<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> I2_HD __forceinline__ T vlen(v3t<T> a) { return sqrt(dot(a, a)); }
+template <class T> I2_HD __forceinline__ v3t<T> unit(v3t<T> a) { const T inv = T(1.0) / vlen(a); return {a.x * inv, a.y * inv, a.z * inv}; }
+template <class T> I2_HD __forceinline__ v3t<T> divs(v3t<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
 
 // (Psi, Theta) accumulator
-struct q4 { double x, y, z, w; };
+template <class T> struct q4t { T x, y, z, w; };
+using q4 = q4t<double>;
 I2_HD __forceinline__ q4 q4zero() { return {0.0, 0.0, 0.0, 0.0}; }
-I2_HD __forceinline__ q4 operator+(q4 a, q4 b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
-I2_HD __forceinline__ q4 operator-(q4 a, q4 b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
-I2_HD __forceinline__ q4 operator*(double s, q4 a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+template <class T> I2_HD __forceinline__ q4t<T> operator+(q4t<T> a, q4t<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+template <class T> I2_HD __forceinline__ q4t<T> operator-(q4t<T> a, q4t<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; }
+template <class S, class T> I2_HD __forceinline__ q4t<T> operator*(S s, q4t<T> a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
 
 // Per-panel record: vertices A, B, C (reference vertex order), the edge
 // tangents of the inner integral (taua = (C-B)/|C-B|, taub = (A-C)/|A-C|,
@@ -82,12 +91,12 @@ I2_HD __forceinline__ double sign_db(double x) {
 }
 I2_HD __forceinline__ double arg_db(double x) { return (x > DOUBLE_MIN) ? 0.0 : PI; }
 // src/common/cuda_math.cu:14-27
-I2_HD __forceinline__ double angle(d3 a, d3 b) {
-    const double den = sqrt(dot(a, a) * dot(b, b));
-    if (den < EPS_ZERO) return 0.0;
-    const double r = dot(a, b) / den;
-    if (r >= 1.0) return 0.0;
-    if (r <= -1.0) return PI;
+template <class T> I2_HD __forceinline__ T angle(v3t<T> a, v3t<T> b) {
+    const T den = sqrt(dot(a, a) * dot(b, b));
+    if (val(den) < EPS_ZERO) return T(0.0);
+    const T r = dot(a, b) / den;
+    if (val(r) >= 1.0) return T(0.0);
+    if (val(r) <= -1.0) return T(PI);
     return acos(r);
 }
 
@@ -154,14 +163,15 @@ I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J) {
 // Edge-sharing pair: vertices of K_j rotated so that JA is off the shared
 // edge (evaluatorJ3DK.cu:797-847), plus the tangents and 1/|JC-JB| that
 // evaluatorJ3DK.cu:338-343 recompute per point.
-struct EdgeCtx { d3 JB, JC, taua, taub, tauc; double ilvc; };
-I2_HD __forceinline__ EdgeCtx edge_ctx(d3 JA, d3 JB, d3 JC) {
-    EdgeCtx c;
+template <class T> struct EdgeCtxT { v3t<T> JB, JC, taua, taub, tauc; T ilvc; };
+using EdgeCtx = EdgeCtxT<double>;
+template <class T> I2_HD __forceinline__ EdgeCtxT<T> edge_ctx(v3t<T> JA, v3t<T> JB, v3t<T> JC) {
+    EdgeCtxT<T> c;
     c.JB = JB; c.JC = JC;
     c.taua = unit(JA - JC);
     c.taub = unit(JB - JA);
-    d3 tc = JC - JB;
-    c.ilvc = 1.0 / vlen(tc);
+    v3t<T> tc = JC - JB;
+    c.ilvc = T(1.0) / vlen(tc);
     c.tauc = c.ilvc * tc;
     return c;
 }
@@ -184,19 +194,21 @@ I2_HD __forceinline__ q4 sing_edge(d3 p, const EdgeCtx& c) {
 
 // Vertex-sharing pair: JA is the shared vertex; e and the two delta angles
 // follow evaluatorJ3DK.cu:376-399 (two sign flips of e).
-struct VertexCtx { d3 JA, taua, taub, e; double invAri; };
-I2_HD __forceinline__ d3 vertex_e(d3 nI, d3 nJ, d3 taua, d3 taub, double* dA, double* dB) {
-    d3 e = cross(nI, nJ);
-    if (dot(e, e) < EPS_ZERO2) e = taub;
+template <class T> struct VertexCtxT { v3t<T> JA, taua, taub, e; T invAri; };
+using VertexCtx = VertexCtxT<double>;
+template <class T>
+I2_HD __forceinline__ v3t<T> vertex_e(v3t<T> nI, v3t<T> nJ, v3t<T> taua, v3t<T> taub, T* dA, T* dB) {
+    v3t<T> e = cross(nI, nJ);
+    if (val(dot(e, e)) < EPS_ZERO2) e = taub;
     else e = unit(e);
-    double dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
-    double dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
-    if ((PI - fabs(dx) < EPS_ZERO) || (PI - fabs(dy) < EPS_ZERO)) {
+    T dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
+    T dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
+    if ((PI - fabs(val(dx)) < EPS_ZERO) || (PI - fabs(val(dy)) < EPS_ZERO)) {
         e = -e;
         dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
         dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
     }
-    if ((dx * dy < 0) && (fabs(dx - dy) > PI)) {
+    if ((val(dx) * val(dy) < 0) && (fabs(val(dx) - val(dy)) > PI)) {
         e = -e;
         dx = atan2(dot(cross(taua, e), nJ), -dot(e, taua));
         dy = atan2(dot(cross(e, taub), nJ), dot(e, taub));
@@ -204,14 +216,15 @@ I2_HD __forceinline__ d3 vertex_e(d3 nI, d3 nJ, d3 taua, d3 taub, double* dA, do
     *dA = dx; *dB = dy;
     return e;
 }
-I2_HD __forceinline__ VertexCtx vertex_ctx(d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
-    VertexCtx c;
+template <class T>
+I2_HD __forceinline__ VertexCtxT<T> vertex_ctx(v3t<T> JA, v3t<T> JB, v3t<T> JC, v3t<T> nI, v3t<T> nJ, T areaI) {
+    VertexCtxT<T> c;
     c.JA = JA;
     c.taua = unit(JA - JC);
     c.taub = unit(JB - JA);
-    double dA, dB;
+    T dA, dB;
     c.e = vertex_e(nI, nJ, c.taua, c.taub, &dA, &dB);
-    c.invAri = 1.0 / sqrt(areaI);   // rsqrt(measures[i]), evaluatorJ3DK.cu:401
+    c.invAri = T(1.0) / sqrt(areaI);   // rsqrt(measures[i]), evaluatorJ3DK.cu:401
     return c;
 }
 // evaluatorJ3DK.cu:352-407
@@ -261,15 +274,15 @@ I2_HD __forceinline__ void shifts_edge(const Panel& I, const Panel& J, int* s1, 
 
 // ------------------------------------------------------------ analytic singular integrals
 // evaluatorJ3DK.cu:418-421
-I2_HD __forceinline__ double phi_f(double sA, double cA, double sG, double cG, double sXi, double cosLambda) {
+template <class T> I2_HD __forceinline__ T phi_f(T sA, T cA, T sG, T cG, T sXi, T cosLambda) {
     return 2.0 * atan2(sXi * sA * sG, 1.0 - cA + cG + cosLambda);
 }
 // evaluatorJ3DK.cu:438-461 -> (q_Theta, q_Psi)
-I2_HD inline void q_theta_psi(double sA, double cA, double sB, double cB, double sG, double cG, double sNu, double cNu,
-                                   double sXi, double cXi, double cosMu, double cosLambda, double* qT, double* qP) {
-    const double phi1 = phi_f(sA, cA, sG, cG, sXi, cosLambda);
-    const double phi2 = phi_f(sA, -cA, sG, -cG, sXi, cosLambda);
-    const double iS = 1.0 / (sA * (1.0 - cosMu * cosMu));
+template <class T>
+I2_HD inline void q_theta_psi(T sA, T cA, T sB, T cB, T sG, T cG, T sNu, T cNu, T sXi, T cXi, T cosMu, T cosLambda, T* qT, T* qP) {
+    const T phi1 = phi_f(sA, cA, sG, cG, sXi, cosLambda);
+    const T phi2 = phi_f(sA, -cA, sG, -cG, sXi, cosLambda);
+    const T iS = 1.0 / (sA * (1.0 - cosMu * cosMu));
     *qT = phi1 + sG * sNu * iS * ((cB * sG - cXi * sB * cG) * phi2 +
                                   sXi * sB * (0.5 * (1.0 + cosMu) * log((1.0 + cB) / (1.0 - cNu)) +
                                               0.5 * (1.0 - cosMu) * log((1.0 - cB) / (1.0 + cNu)) +
@@ -281,106 +294,107 @@ I2_HD inline void q_theta_psi(double sA, double cA, double sB, double cB, double
                       phi2 * sXi * sG * sNu * sB);
 }
 // evaluatorJ3DK.cu:474-481 (coplanar case; q_Theta takes its documented value 0)
-I2_HD inline void q_theta_psi_zero(double sB, double cB, double sNu, double cNu, double sinAlpha, double* qT, double* qP) {
-    *qT = 0.0;
+template <class T> I2_HD inline void q_theta_psi_zero(T sB, T cB, T sNu, T cNu, T sinAlpha, T* qT, T* qP) {
+    *qT = T(0.0);
     *qP = 1.5 - (cNu * sB * log(1.0 + cNu) + sNu * cB * log(1.0 - cB) + sinAlpha - sB + sNu + sB * cNu * log(sB / sNu)) / sinAlpha;
 }
 
 // evaluatorJ3DK.cu:603-672: closed-form integral over K_i of the edge-sharing singular part.
 // (IA, IB, IC) and (JA, JB, JC) are rotated so that IA and JA are off the shared edge.
-I2_HD inline q4 integrate_sing_edge(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
-    const d3 taua = unit(JA - JC), taub = unit(JB - JA), tauc = unit(JC - JB);
-    const double alpha = angle(IA - IC, IB - IC);
-    const double beta = angle(IC - IB, IA - IB);
-    const double gamma = angle(JC - JB, JA - JB);
-    const double delta = angle(JB - JC, JA - JC);
-    const double nu = PI - alpha - beta;
-    const double xi = atan2(dot(cross(nI, nJ), tauc), dot(nI, nJ));
-    double sA, cA, sB, cB, sG, cG, sD, cD, sX, cX;
+template <class T>
+I2_HD inline q4t<T> integrate_sing_edge(v3t<T> IA, v3t<T> IB, v3t<T> IC, v3t<T> JA, v3t<T> JB, v3t<T> JC, v3t<T> nI,
+                                        v3t<T> nJ, T areaI) {
+    const v3t<T> taua = unit(JA - JC), taub = unit(JB - JA), tauc = unit(JC - JB);
+    const T alpha = angle(IA - IC, IB - IC);
+    const T beta = angle(IC - IB, IA - IB);
+    const T gamma = angle(JC - JB, JA - JB);
+    const T delta = angle(JB - JC, JA - JC);
+    const T nu = PI - alpha - beta;
+    const T xi = atan2(dot(cross(nI, nJ), tauc), dot(nI, nJ));
+    T sA, cA, sB, cB, sG, cG, sD, cD, sX, cX;
     sincos(alpha, &sA, &cA); sincos(beta, &sB, &cB); sincos(gamma, &sG, &cG);
     sincos(delta, &sD, &cD); sincos(xi, &sX, &cX);
-    const double sNu = sA * cB + cA * sB;
-    const double cNu = sA * sB - cA * cB;
-    const double cosSigma = -(cA * cD + cX * sA * sD);
-    const double cosMu = -(cB * cG + cX * sB * sG);
-    const double cosLambda = -(cA * cG - cX * sA * sG);
-    const double cosTheta = -(cB * cD - cX * sB * sD);
-    const double qab = sNu * log(tan(0.5 * alpha) * tan(0.5 * nu)) / sB +
+    const T sNu = sA * cB + cA * sB;
+    const T cNu = sA * sB - cA * cB;
+    const T cosSigma = -(cA * cD + cX * sA * sD);
+    const T cosMu = -(cB * cG + cX * sB * sG);
+    const T cosLambda = -(cA * cG - cX * sA * sG);
+    const T cosTheta = -(cB * cD - cX * sB * sD);
+    const T qab = sNu * log(tan(0.5 * alpha) * tan(0.5 * nu)) / sB +
                        sNu * log(tan(0.5 * beta) * tan(0.5 * nu)) / sA +
                        log(tan(0.5 * alpha) * tan(0.5 * beta));
-    double qAT, qAP, qBT, qBP;
-    if ((fabs(xi) < EPS_ZERO) && (fabs(beta - gamma) < EPS_ZERO)) q_theta_psi_zero(sB, cB, sNu, cNu, sA, &qAT, &qAP);
+    T qAT, qAP, qBT, qBP;
+    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(beta) - val(gamma)) < EPS_ZERO)) q_theta_psi_zero(sB, cB, sNu, cNu, sA, &qAT, &qAP);
     else q_theta_psi(sA, cA, sB, cB, sG, cG, sNu, cNu, sX, cX, cosMu, cosLambda, &qAT, &qAP);
-    if ((fabs(xi) < EPS_ZERO) && (fabs(alpha - delta) < EPS_ZERO)) q_theta_psi_zero(sA, cA, sNu, cNu, sB, &qBT, &qBP);
+    if ((fabs(val(xi)) < EPS_ZERO) && (fabs(val(alpha) - val(delta)) < EPS_ZERO)) q_theta_psi_zero(sA, cA, sNu, cNu, sB, &qBT, &qBP);
     else q_theta_psi(sB, cB, sA, cA, sD, cD, sNu, cNu, sX, cX, cosSigma, cosTheta, &qBT, &qBP);
-    q4 r;
-    const d3 v = areaI * (qAP * taub + qBP * taua - qab * tauc);
+    q4t<T> r;
+    const v3t<T> v = areaI * (qAP * taub + qBP * taua - qab * tauc);
     r.x = v.x; r.y = v.y; r.z = v.z;
     r.w = areaI * (qAT + qBT);
     return r;
 }
 
 // evaluatorJ3DK.cu:511-601 -> (q^Theta, q^Psi) for vertex-sharing pairs
-I2_HD inline void q_theta_psi_cont(double sXi, double cXi, double mu, double sMu, double cMu, double logSinMu, double logSinNu,
-                                        double psi, double sPsi, double cPsi, double nu, double sNu, double cNu, double kappa,
-                                        double sinKappa, double sinNuPsi, double sinMuPsi, double delta, double sD, double cD,
-                                        double cosLambda, double cosTheta, double cosEta, double cosSigma, double cosChi,
-                                        double* qT, double* qP) {
-    const double logOneCosTheta = log(1.0 + cosTheta);
-    const double logOneCosLambda = log(1.0 + cosLambda);
-    const double Lambda1 = logOneCosLambda - logOneCosTheta + logSinNu - logSinMu;
-    const double Lambda2 = log(tan(0.5 * nu) * tan(0.5 * mu));
-    double s05D, c05D;
+template <class T>
+I2_HD inline void q_theta_psi_cont(T sXi, T cXi, T mu, T sMu, T cMu, T logSinMu, T logSinNu, T psi, T sPsi, T cPsi, T nu,
+                                   T sNu, T cNu, T kappa, T sinKappa, T sinNuPsi, T sinMuPsi, T delta, T sD, T cD,
+                                   T cosLambda, T cosTheta, T cosEta, T cosSigma, T cosChi, T* qT, T* qP) {
+    const T logOneCosTheta = log(1.0 + cosTheta);
+    const T logOneCosLambda = log(1.0 + cosLambda);
+    const T Lambda1 = logOneCosLambda - logOneCosTheta + logSinNu - logSinMu;
+    const T Lambda2 = log(tan(0.5 * nu) * tan(0.5 * mu));
+    T s05D, c05D;
     sincos(0.5 * delta, &s05D, &c05D);
-    const double tan05Delta = s05D / c05D;
-    double sMP, cMP, sNP, cNP;
+    const T tan05Delta = s05D / c05D;
+    T sMP, cMP, sNP, cNP;
     sincos(0.5 * (mu - psi), &sMP, &cMP);
     sincos(0.5 * (nu + psi), &sNP, &cNP);
-    const double Amu = atan2(tan05Delta * cMP * sXi, tan05Delta * cMP * cXi + sMP);
-    const double Anu = atan2(tan05Delta * sNP * sXi, tan05Delta * sNP * cXi + cNP);
-    double sT, cT, sT2, cT2;
+    const T Amu = atan2(tan05Delta * cMP * sXi, tan05Delta * cMP * cXi + sMP);
+    const T Anu = atan2(tan05Delta * sNP * sXi, tan05Delta * sNP * cXi + cNP);
+    T sT, cT, sT2, cT2;
     sincos(0.5 * (mu - psi) - 0.5 * (nu + psi), &sT, &cT);
     sincos(0.5 * kappa, &sT2, &cT2);
-    const double W = atan2(sD * sT2 * sXi, cT2 + sD * cT * cXi + cD * sT);
+    const T W = atan2(sD * sT2 * sXi, cT2 + sD * cT * cXi + cD * sT);
     sincos(delta - psi, &sT, &cT);
-    const double D = 1.0 / (sT * sT + sD * sPsi * (1.0 - cXi) * (cT + cosSigma));
-    const double G = cPsi * (sD * cosSigma * cXi + cD * cosChi - sD * sD / sPsi);
-    const double gent = 2.0 * (Anu * sMu * sinNuPsi - Amu * sNu * sinMuPsi -
+    const T D = 1.0 / (sT * sT + sD * sPsi * (1.0 - cXi) * (cT + cosSigma));
+    const T G = cPsi * (sD * cosSigma * cXi + cD * cosChi - sD * sD / sPsi);
+    const T gent = 2.0 * (Anu * sMu * sinNuPsi - Amu * sNu * sinMuPsi -
                                D * sMu * sNu * sD * (W * cosEta + 0.5 * sPsi * sXi * (Lambda1 - Lambda2 * cosSigma))) /
                         (sPsi * sinKappa);
-    const double gens = 0.5 * (3.0 - LOG2) +
+    const T gens = 0.5 * (3.0 - LOG2) +
                         (sMu * sNu / sinKappa) * ((logOneCosLambda - logOneCosTheta) * cPsi / sPsi +
                                                   D * (Lambda1 * sD * cosEta / sPsi + Lambda2 * cosChi -
                                                        2.0 * W * sD * sXi - G * (logSinNu - logSinMu))) -
                         (cMu * sNu * (logOneCosLambda - logSinMu) + sMu * cNu * (logOneCosTheta - logSinNu)) / sinKappa -
                         0.5 * (logSinMu + logSinNu - log(sinKappa));
-    const double mulPsi = sign_db(sXi * sPsi);
-    const double mulDelta = sign_db(sXi * sD);
-    if ((fabs(sXi) < EPS_ZERO) && (1.0 - fabs(cosSigma) < 0.5 * EPS_ZERO2) && (fabs(sPsi) > EPS_ZERO)) {
-        const double ara = arg_db(sin(0.5 * (nu + psi)) * cosSigma);
-        const double arb = arg_db(cos(0.5 * (mu - psi)) * cosSigma);
+    const double mulPsi = sign_db(val(sXi * sPsi));
+    const double mulDelta = sign_db(val(sXi * sD));
+    if ((fabs(val(sXi)) < EPS_ZERO) && (1.0 - fabs(val(cosSigma)) < 0.5 * EPS_ZERO2) && (fabs(val(sPsi)) > EPS_ZERO)) {
+        const double ara = arg_db(val(sin(0.5 * (nu + psi)) * cosSigma));
+        const double arb = arg_db(val(cos(0.5 * (mu - psi)) * cosSigma));
         *qT = 2.0 * mulPsi * cosSigma * cXi / (sinKappa * sPsi) * (sMu * sinNuPsi * ara - sNu * sinMuPsi * arb);
         *qP = 0.5 * (1.0 - LOG2) - 0.5 * log((1.0 - cosSigma * cMu) * (1.0 + cosSigma * cNu) / sinKappa) +
               cosSigma * (sMu - sNu + 0.5 * sin(mu - nu) * Lambda2) / sinKappa;
         return;
     }
-    if ((fabs(sXi) < EPS_ZERO) && (fabs(sPsi) > EPS_ZERO)) {
-        const double are = arg_db(sin(0.5 * (nu + mu)) + sin(0.5 * (mu - nu) - psi + delta * cXi));
-        const double arc = arg_db(1.0 / tan(0.5 * (nu + psi)) + tan(0.5 * delta) * cXi);
-        const double ard = arg_db(tan(0.5 * (mu - psi)) + tan(0.5 * delta) * cXi);
+    if ((fabs(val(sXi)) < EPS_ZERO) && (fabs(val(sPsi)) > EPS_ZERO)) {
+        const double are = arg_db(val(sin(0.5 * (nu + mu)) + sin(0.5 * (mu - nu) - psi + delta * cXi)));
+        const double arc = arg_db(val(1.0 / tan(0.5 * (nu + psi)) + tan(0.5 * delta) * cXi));
+        const double ard = arg_db(val(tan(0.5 * (mu - psi)) + tan(0.5 * delta) * cXi));
         *qT = 2.0 * mulDelta * (sD * sMu * sNu / cosChi * cXi * are + sMu * sinNuPsi * arc - sNu * sinMuPsi * ard) / (sinKappa * sPsi);
         *qP = gens;
         return;
     }
-    if (fabs(sin(psi)) < EPS_ZERO) {
-        if (fabs(delta) > EPS_ZERO) {
+    if (fabs(val(sin(psi))) < EPS_ZERO) {
+        if (fabs(val(delta)) > EPS_ZERO) {
             *qT = 2.0 * (sMu * sNu * (W * cD * cXi - 0.5 * (Lambda1 - Lambda2 * cosSigma) * sXi) / sD +
-                         (Anu + mulDelta * arg_db(sin(0.5 * (nu + psi)))) * sMu * cNu +
-                         (Amu + mulDelta * arg_db(cos(0.5 * (mu - psi)))) * cMu * sNu) / sinKappa;
+                         (Anu + mulDelta * arg_db(val(sin(0.5 * (nu + psi))))) * sMu * cNu +
+                         (Amu + mulDelta * arg_db(val(cos(0.5 * (mu - psi))))) * cMu * sNu) / sinKappa;
             *qP = 0.5 * (3.0 - LOG2) - 0.5 * (log((1.0 + cosLambda) * (1.0 + cosTheta) / sinKappa) - sin(mu - nu) / sinKappa * Lambda1) -
                   sMu * sNu * ((Lambda1 * cD - Lambda2 * cPsi) * cXi + 2.0 * W * sXi) / (sinKappa * sD);
         } else {
-            *qT = 2.0 * arg_db(cPsi);
+            *qT = T(2.0 * arg_db(val(cPsi)));
             *qP = 0.5 * (1.0 - LOG2) - 0.5 * log((1.0 - cPsi * cMu) * (1.0 + cPsi * cNu) / sinKappa) +
                   (0.5 * sin(mu - nu) * Lambda1 + cPsi * (sMu - sNu)) / sinKappa;
         }
@@ -392,65 +406,75 @@ I2_HD inline void q_theta_psi_cont(double sXi, double cXi, double mu, double sMu
 
 // evaluatorJ3DK.cu:674-780: closed-form integral over K_i of the vertex-sharing
 // singular part.  IA and JA are the shared vertex.
-I2_HD inline q4 integrate_sing_vertex(d3 IA, d3 IB, d3 IC, d3 JA, d3 JB, d3 JC, d3 nI, d3 nJ, double areaI) {
-    const d3 taua = unit(JA - JC), taub = unit(JB - JA);
-    double dA, dB;
-    const d3 e = vertex_e(nI, nJ, taua, taub, &dA, &dB);
-    const double xi = atan2(dot(cross(nI, nJ), e), dot(nI, nJ));
-    double sDA, cDA, sDB, cDB, sX, cX;
+template <class T>
+I2_HD inline q4t<T> integrate_sing_vertex(v3t<T> IA, v3t<T> IB, v3t<T> IC, v3t<T> JA, v3t<T> JB, v3t<T> JC, v3t<T> nI,
+                                          v3t<T> nJ, T areaI) {
+    const v3t<T> taua = unit(JA - JC), taub = unit(JB - JA);
+    T dA, dB;
+    const v3t<T> e = vertex_e(nI, nJ, taua, taub, &dA, &dB);
+    const T xi = atan2(dot(cross(nI, nJ), e), dot(nI, nJ));
+    T sDA, cDA, sDB, cDB, sX, cX;
     sincos(dA, &sDA, &cDA); sincos(dB, &sDB, &cDB); sincos(xi, &sX, &cX);
-    const d3 s = IC - IB;
-    const double nu = angle(IA - IB, IC - IB);
-    const double mu = angle(IB - IC, IA - IC);
-    const double kappa = angle(IB - IA, IC - IA);
-    double sMu, cMu, sNu, cNu;
+    const v3t<T> s = IC - IB;
+    const T nu = angle(IA - IB, IC - IB);
+    const T mu = angle(IB - IC, IA - IC);
+    const T kappa = angle(IB - IA, IC - IA);
+    T sMu, cMu, sNu, cNu;
     sincos(mu, &sMu, &cMu); sincos(nu, &sNu, &cNu);
-    const double logSinNu = log(sNu), logSinMu = log(sMu);
-    const double sinKappa = sin(kappa);
-    const double psi = atan2(dot(cross(e, s), nI), dot(e, s));
-    double sPsi, cPsi, sNP, cNP, sMP, cMP;
+    const T logSinNu = log(sNu), logSinMu = log(sMu);
+    const T sinKappa = sin(kappa);
+    const T psi = atan2(dot(cross(e, s), nI), dot(e, s));
+    T sPsi, cPsi, sNP, cNP, sMP, cMP;
     sincos(psi, &sPsi, &cPsi);
     sincos(nu + psi, &sNP, &cNP);
     sincos(mu - psi, &sMP, &cMP);
-    double qAT, qAP, qBT, qBP;
+    T qAT, qAP, qBT, qBP;
     {
-        const double cosSigma = sDA * sPsi * cX + cDA * cPsi;
-        const double cosChi = sDA * cPsi * cX - cDA * sPsi;
-        const double cosEta = cDA * sPsi * cX - sDA * cPsi;
-        const double cosTheta = sDA * sNP * cX + cDA * cNP;
-        const double cosLambda = sDA * sMP * cX - cDA * cMP;
+        const T cosSigma = sDA * sPsi * cX + cDA * cPsi;
+        const T cosChi = sDA * cPsi * cX - cDA * sPsi;
+        const T cosEta = cDA * sPsi * cX - sDA * cPsi;
+        const T cosTheta = sDA * sNP * cX + cDA * cNP;
+        const T cosLambda = sDA * sMP * cX - cDA * cMP;
         q_theta_psi_cont(sX, cX, mu, sMu, cMu, logSinMu, logSinNu, psi, sPsi, cPsi, nu, sNu, cNu, kappa, sinKappa,
                          sNP, sMP, dA, sDA, cDA, cosLambda, cosTheta, cosEta, cosSigma, cosChi, &qAT, &qAP);
     }
     {
-        const double cosSigma = sDB * sPsi * cX + cDB * cPsi;
-        const double cosChi = sDB * cPsi * cX - cDB * sPsi;
-        const double cosEta = cDB * sPsi * cX - sDB * cPsi;
-        const double cosTheta = sDB * sNP * cX + cDB * cNP;
-        const double cosLambda = sDB * sMP * cX - cDB * cMP;
+        const T cosSigma = sDB * sPsi * cX + cDB * cPsi;
+        const T cosChi = sDB * cPsi * cX - cDB * sPsi;
+        const T cosEta = cDB * sPsi * cX - sDB * cPsi;
+        const T cosTheta = sDB * sNP * cX + cDB * cNP;
+        const T cosLambda = sDB * sMP * cX - cDB * cMP;
         q_theta_psi_cont(sX, cX, mu, sMu, cMu, logSinMu, logSinNu, psi, sPsi, cPsi, nu, sNu, cNu, kappa, sinKappa,
                          sNP, sMP, dB, sDB, cDB, cosLambda, cosTheta, cosEta, cosSigma, cosChi, &qBT, &qBP);
     }
-    q4 r;
-    const d3 v = areaI * (qAP * taua + qBP * taub);
+    q4t<T> r;
+    const v3t<T> v = areaI * (qAP * taua + qBP * taub);
     r.x = v.x; r.y = v.y; r.z = v.z;
-    r.w = (fabs(xi) < EPS_ZERO) ? 0.0 : (areaI * (qAT - qBT));
+    r.w = (fabs(val(xi)) < EPS_ZERO) ? T(0.0) : (areaI * (qAT - qBT));
     return r;
 }
 
 // ------------------------------------------------------------ finalize
 // evaluatorJ3DK.cu:224-241 (vertex-sharing, with the 2*pi*p wrap of Theta)
-I2_HD __forceinline__ d3 finalize_vertex(q4 I, d3 nJ, double areaI) {
+// wrap count p of Theta' = Theta + 2 p (2 pi S_i) into [-2 pi S_i, 2 pi S_i]
+I2_HD __forceinline__ int wrap_count(double theta, double areaI) {
     int p = 0;
     const double refTheta = TWO_PI * areaI;
-    if (I.w > refTheta) p = -((int)trunc((I.w - refTheta) / (2.0 * refTheta)) + 1);
-    else if (I.w < -refTheta) p = ((int)trunc((-refTheta - I.w) / (2.0 * refTheta)) + 1);
-    const d3 psi = {I.x, I.y, I.z};
+    if (theta > refTheta) p = -((int)trunc((theta - refTheta) / (2.0 * refTheta)) + 1);
+    else if (theta < -refTheta) p = ((int)trunc((-refTheta - theta) / (2.0 * refTheta)) + 1);
+    return p;
+}
+template <class T> I2_HD __forceinline__ v3t<T> finalize_vertex_p(q4t<T> I, v3t<T> nJ, T areaI, int p) {
+    const T refTheta = TWO_PI * areaI;
+    const v3t<T> psi = {I.x, I.y, I.z};
     return RECIP_4PI * ((I.w + 2.0 * p * refTheta) * nJ + cross(psi, nJ));
 }
+I2_HD __forceinline__ d3 finalize_vertex(q4 I, d3 nJ, double areaI) {
+    return finalize_vertex_p(I, nJ, areaI, wrap_count(I.w, areaI));
+}
 // evaluatorJ3DK.cu:254-264
-I2_HD __forceinline__ d3 finalize_plain(q4 I, d3 nJ) {
-    const d3 psi = {I.x, I.y, I.z};
+template <class T> I2_HD __forceinline__ v3t<T> finalize_plain(q4t<T> I, v3t<T> nJ) {
+    const v3t<T> psi = {I.x, I.y, I.z};
     return RECIP_4PI * (I.w * nJ + cross(psi, nJ));
 }
 

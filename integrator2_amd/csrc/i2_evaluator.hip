@@ -12,6 +12,7 @@
 // a fixed order, so results are bitwise reproducible.
 #include "i2_internal.hpp"
 #include "i2_batch.hpp"
+#include "i2_adjoint.hpp"
 
 #include <algorithm>
 #include <fstream>
@@ -437,6 +438,7 @@ void i2_evaluator_destroy(i2_evaluator* ev) {
         }
         clear_launches(ev);
         engine_release_evaluator(ev);
+        adjoint_release(ev->adj);
         ev->d_counter.reset();
         if (ev->h_counter) (void)hipHostFree(ev->h_counter);
         if (ev->ev0) (void)hipEventDestroy(ev->ev0);

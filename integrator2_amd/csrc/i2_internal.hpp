@@ -153,6 +153,8 @@ struct LaunchRecord {
     hipEvent_t e0, e1;
 };
 
+namespace i2 { struct AdjState; }
+
 struct i2_evaluator {
     const i2_mesh* mesh = nullptr;
     int path = 0;                  // 0: plain per-type launches, 1: batched persistent kernels
@@ -169,6 +171,7 @@ struct i2_evaluator {
     i2::DBuf<unsigned long long> d_counter;
     unsigned long long* h_counter = nullptr;   // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    i2::AdjState* adj = nullptr;   // adjoint buffers (created by the first loss-and-gradient call)
 };
 
 namespace i2 {

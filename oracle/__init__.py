@@ -32,6 +32,7 @@ def lib():
         L.o_normal_draw.restype = ctypes.c_double
         L.o_normal_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         L.o_loss_tasks.restype = ctypes.c_double
+        L.o_loss_tasks_prec.restype = ctypes.c_double
         L.o_now.restype = ctypes.c_double
         L.o_singular_integral.restype = ctypes.c_uint
         _lib = L
@@ -141,14 +142,16 @@ class Oracle:
                             ctypes.c_int(len(w)), ctypes.c_int(order), ctypes.c_int(level), _ptr(out))
         return out
 
-    def loss_tasks(self, t, ij, rlevel, L12, w, order, seed, vtx=-1, axis=0, h=0.0, dual=False, nthreads=0):
+    def loss_tasks(self, t, ij, rlevel, L12, w, order, seed, vtx=-1, axis=0, h=0.0, dual=False, nthreads=0,
+                   long_double=False):
         ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
         rlevel = np.ascontiguousarray(rlevel, np.int8)
         L12 = np.ascontiguousarray(L12, np.float64)
         w = np.ascontiguousarray(w, np.float64)
-        return lib().o_loss_tasks(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12), _ptr(w),
-                                  ctypes.c_int(len(w)), ctypes.c_int(order), ctypes.c_uint64(seed), ctypes.c_int(vtx),
-                                  ctypes.c_int(axis), _D(h), ctypes.c_int(int(dual)), ctypes.c_int(nthreads))
+        return lib().o_loss_tasks_prec(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12),
+                                       _ptr(w), ctypes.c_int(len(w)), ctypes.c_int(order), ctypes.c_uint64(seed),
+                                       ctypes.c_int(vtx), ctypes.c_int(axis), _D(h), ctypes.c_int(int(dual)),
+                                       ctypes.c_int(int(long_double)), ctypes.c_int(nthreads))
 
 
 def symmetry(results3):

@@ -371,19 +371,53 @@ void o_symmetry(const double* results3, int64_t n, double* out_delta) {
 
 double o_normal_draw(uint64_t seed, uint64_t pair_id, uint64_t stream) { return normal_draw(seed, pair_id, stream); }
 
+}  // extern "C"
+
+namespace {
+// Loss term w Theta + v . J of task (i, j) at level rl with the wrap count p,
+// geometry in scalar T with vertex `vtx` replaced by *moved (vtx < 0: none).
+template <class T>
+T task_loss(const Mesh& m, const Rule& R, int type, int i, int j, int rl, int p, const V3<T>* moved, int vtx,
+            double wq, const double vq[3]) {
+    TaskGeom<T> g = task_geom<T>(m, i, j, moved, vtx);
+    Q4<T> v = task_level_value(m, R, type, i, j, rl, g) + singular_integral(m, type, i, j, g);
+    V3<T> J;
+    if (type == VERTEX_SHARING) {
+        const T refT = T(TWO_PI) * g.areaI;
+        J = T(RECIPROCAL_FOUR_PI) * ((v.w + T(2.0 * p) * refT) * g.nJ + cross(vecof(v), g.nJ));
+    } else J = finalize_plain(v, g.nJ);
+    return T(wq) * v.w + T(vq[0]) * J.x + T(vq[1]) * J.y + T(vq[2]) * J.z;
+}
+template <class R_> V3<R_> moved_vertex(const Mesh& m, int vtx, int axis, R_ h) {
+    V3<R_> x{R_(m.v[vtx].x), R_(m.v[vtx].y), R_(m.v[vtx].z)};
+    if (axis == 0) x.x += h; else if (axis == 1) x.y += h; else x.z += h;
+    return x;
+}
+template <class R_> V3<DualT<R_>> seeded_vertex(const Mesh& m, int vtx, int axis) {
+    V3<DualT<R_>> x{DualT<R_>(R_(m.v[vtx].x)), DualT<R_>(R_(m.v[vtx].y)), DualT<R_>(R_(m.v[vtx].z))};
+    if (axis == 0) x.x.d = 1; else if (axis == 1) x.y.d = 1; else x.z.d = 1;
+    return x;
+}
+}  // namespace
+
+extern "C" {
+
 // Loss contribution of ordered tasks (i_q, j_q) of one type, each at its
 // frozen reported level rlevel[q], with vertex `vtx` moved by `h` along
 // `axis` (vtx < 0: no move).  Weights: w = N(0,1) stream 0, v = N(0,1)^3
 // streams 1..3, pair id = i*nT + j.  The 2*pi*p wrap of vertex-sharing
 // tasks is frozen at its unperturbed value.  dual != 0 returns dL/dx of the
-// same sum (forward mode) instead of its value.
-double o_loss_tasks(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n,
-                    const double* L12, const double* w, int nPts, int order, uint64_t seed,
-                    int vtx, int axis, double hstep, int dual, int nthreads) {
+// same sum (forward mode) instead of its value.  prec = 1 evaluates the
+// geometry and formulas in long double (64-bit significand): the finite
+// differences of the adjoint's check use it so that their rounding noise,
+// divided by the step, stays far below the gate.
+double o_loss_tasks_prec(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n,
+                         const double* L12, const double* w, int nPts, int order, uint64_t seed,
+                         int vtx, int axis, double hstep, int dual, int prec, int nthreads) {
     const Mesh& m = *(Mesh*)h;
     Rule R = make_rule(L12, w, nPts, order);
     if (nthreads > 0) omp_set_num_threads(nthreads);
-    double total = 0.0;
+    long double total = 0.0L;
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
     for (int64_t q = 0; q < n; ++q) {
         const int i = ij[2 * q], j = ij[2 * q + 1];
@@ -395,36 +429,28 @@ double o_loss_tasks(void* h, int type, const int32_t* ij, const int8_t* rlevel, 
         Q4<double> v0 = task_level_value(m, R, type, i, j, rlevel[q], g0) + singular_integral(m, type, i, j, g0);
         int p = 0;
         if (type == VERTEX_SHARING) finalize_vertex(v0, g0.nJ, g0.areaI, &p);
-        if (!dual) {
-            V3<double> moved{0, 0, 0};
-            const V3<double>* ov = nullptr;
-            if (vtx >= 0) {
-                moved = m.v[vtx];
-                if (axis == 0) moved.x += hstep; else if (axis == 1) moved.y += hstep; else moved.z += hstep;
-                ov = &moved;
-            }
-            TaskGeom<double> g = task_geom<double>(m, i, j, ov, vtx);
-            Q4<double> v = task_level_value(m, R, type, i, j, rlevel[q], g) + singular_integral(m, type, i, j, g);
-            V3<double> J;
-            if (type == VERTEX_SHARING) {
-                const double refT = TWO_PI * g.areaI;
-                J = RECIPROCAL_FOUR_PI * ((v.w + 2.0 * p * refT) * g.nJ + cross(vecof(v), g.nJ));
-            } else J = finalize_plain(v, g.nJ);
-            total += wq * v.w + vq[0] * J.x + vq[1] * J.y + vq[2] * J.z;
+        const int rl = rlevel[q];
+        if (!dual && !prec) {
+            V3<double> mv = vtx >= 0 ? moved_vertex<double>(m, vtx, axis, hstep) : V3<double>{0, 0, 0};
+            total += task_loss<double>(m, R, type, i, j, rl, p, vtx >= 0 ? &mv : nullptr, vtx, wq, vq);
+        } else if (!dual) {
+            V3<long double> mv = vtx >= 0 ? moved_vertex<long double>(m, vtx, axis, (long double)hstep)
+                                          : V3<long double>{0, 0, 0};
+            total += task_loss<long double>(m, R, type, i, j, rl, p, vtx >= 0 ? &mv : nullptr, vtx, wq, vq);
+        } else if (!prec) {
+            V3<Dual> mv = seeded_vertex<double>(m, vtx, axis);
+            total += task_loss<Dual>(m, R, type, i, j, rl, p, &mv, vtx, wq, vq).d;
         } else {
-            V3<Dual> moved{Dual(m.v[vtx].x), Dual(m.v[vtx].y), Dual(m.v[vtx].z)};
-            if (axis == 0) moved.x.d = 1.0; else if (axis == 1) moved.y.d = 1.0; else moved.z.d = 1.0;
-            TaskGeom<Dual> g = task_geom<Dual>(m, i, j, &moved, vtx);
-            Q4<Dual> v = task_level_value(m, R, type, i, j, rlevel[q], g) + singular_integral(m, type, i, j, g);
-            V3<Dual> J;
-            if (type == VERTEX_SHARING) {
-                const Dual refT = Dual(TWO_PI) * g.areaI;
-                J = Dual(RECIPROCAL_FOUR_PI) * ((v.w + Dual(2.0 * p) * refT) * g.nJ + cross(vecof(v), g.nJ));
-            } else J = finalize_plain(v, g.nJ);
-            total += wq * v.w.d + vq[0] * J.x.d + vq[1] * J.y.d + vq[2] * J.z.d;
+            V3<DualL> mv = seeded_vertex<long double>(m, vtx, axis);
+            total += task_loss<DualL>(m, R, type, i, j, rl, p, &mv, vtx, wq, vq).d;
         }
     }
-    return total;
+    return (double)total;
+}
+double o_loss_tasks(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n,
+                    const double* L12, const double* w, int nPts, int order, uint64_t seed,
+                    int vtx, int axis, double hstep, int dual, int nthreads) {
+    return o_loss_tasks_prec(h, type, ij, rlevel, n, L12, w, nPts, order, seed, vtx, axis, hstep, dual, 0, nthreads);
 }
 
 // Wall-clock helper for the CPU baseline.

@@ -50,41 +50,45 @@ constexpr double TWO_PI = 6.28318530717958647692;
 constexpr double RECIPROCAL_FOUR_PI = 0.079577471545947667884;
 
 // ---------------------------------------------------------------- dual numbers
-// Forward-mode dual number: value v and one directional derivative d.
-struct Dual {
-    double v = 0.0, d = 0.0;
-    Dual() = default;
-    Dual(double x) : v(x), d(0.0) {}
-    Dual(double x, double dx) : v(x), d(dx) {}
+// Forward-mode dual number over a real type R (double or long double): value
+// v and one directional derivative d.  Operators are hidden friends so that
+// plain reals convert implicitly.
+template <class R> struct DualT {
+    R v = 0, d = 0;
+    DualT() = default;
+    DualT(R x) : v(x), d(0) {}
+    DualT(R x, R dx) : v(x), d(dx) {}
+    friend DualT operator+(DualT a, DualT b) { return {a.v + b.v, a.d + b.d}; }
+    friend DualT operator-(DualT a, DualT b) { return {a.v - b.v, a.d - b.d}; }
+    friend DualT operator-(DualT a) { return {-a.v, -a.d}; }
+    friend DualT operator*(DualT a, DualT b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+    friend DualT operator/(DualT a, DualT b) { R q = a.v / b.v; return {q, (a.d - q * b.d) / b.v}; }
+    friend DualT& operator+=(DualT& a, DualT b) { a = a + b; return a; }
+    friend DualT& operator-=(DualT& a, DualT b) { a = a - b; return a; }
+    friend DualT& operator*=(DualT& a, DualT b) { a = a * b; return a; }
+    friend DualT& operator/=(DualT& a, DualT b) { a = a / b; return a; }
+    friend bool operator<(DualT a, DualT b) { return a.v < b.v; }
+    friend bool operator>(DualT a, DualT b) { return a.v > b.v; }
+    friend bool operator<=(DualT a, DualT b) { return a.v <= b.v; }
+    friend bool operator>=(DualT a, DualT b) { return a.v >= b.v; }
+    friend DualT sqrt(DualT a) { R s = std::sqrt(a.v); return {s, a.d / (R(2) * s)}; }
+    friend DualT log(DualT a) { return {std::log(a.v), a.d / a.v}; }
+    friend DualT sin(DualT a) { return {std::sin(a.v), a.d * std::cos(a.v)}; }
+    friend DualT cos(DualT a) { return {std::cos(a.v), -a.d * std::sin(a.v)}; }
+    friend DualT tan(DualT a) { R t = std::tan(a.v); return {t, a.d * (R(1) + t * t)}; }
+    friend DualT acos(DualT a) { return {std::acos(a.v), -a.d / std::sqrt(R(1) - a.v * a.v)}; }
+    friend DualT atan2(DualT y, DualT x) {
+        R r2 = x.v * x.v + y.v * y.v;
+        return {std::atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / r2};
+    }
+    friend DualT fabs(DualT a) { return a.v < 0 ? -a : a; }
+    friend R val(const DualT& x) { return x.v; }
 };
-inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
-inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
-inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-inline Dual operator/(Dual a, Dual b) { double q = a.v / b.v; return {q, (a.d - q * b.d) / b.v}; }
-inline Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
-inline Dual& operator-=(Dual& a, Dual b) { a = a - b; return a; }
-inline Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
-inline Dual& operator/=(Dual& a, Dual b) { a = a / b; return a; }
-inline bool operator<(Dual a, Dual b) { return a.v < b.v; }
-inline bool operator>(Dual a, Dual b) { return a.v > b.v; }
-inline bool operator<=(Dual a, Dual b) { return a.v <= b.v; }
-inline bool operator>=(Dual a, Dual b) { return a.v >= b.v; }
-inline Dual sqrt(Dual a) { double s = std::sqrt(a.v); return {s, a.d / (2.0 * s)}; }
-inline Dual log(Dual a) { return {std::log(a.v), a.d / a.v}; }
-inline Dual sin(Dual a) { return {std::sin(a.v), a.d * std::cos(a.v)}; }
-inline Dual cos(Dual a) { return {std::cos(a.v), -a.d * std::sin(a.v)}; }
-inline Dual tan(Dual a) { double t = std::tan(a.v); return {t, a.d * (1.0 + t * t)}; }
-inline Dual acos(Dual a) { return {std::acos(a.v), -a.d / std::sqrt(1.0 - a.v * a.v)}; }
-inline Dual atan2(Dual y, Dual x) {
-    double r2 = x.v * x.v + y.v * y.v;
-    return {std::atan2(y.v, x.v), (x.v * y.d - y.v * x.d) / r2};
-}
-inline Dual fabs(Dual a) { return a.v < 0 ? -a : a; }
+using Dual = DualT<double>;
+using DualL = DualT<long double>;
 
 inline double val(double x) { return x; }
 inline long double val(long double x) { return x; }
-inline double val(const Dual& x) { return x.v; }
 using std::sqrt; using std::log; using std::sin; using std::cos; using std::tan;
 using std::acos; using std::atan2; using std::fabs;
 

@@ -182,3 +182,69 @@ def test_product_refuses_without_device(i2):
     with pytest.raises(i2.I2Error) as ei:
         i2.Mesh3D(0)
     assert "device" in str(ei.value).lower()
+
+
+def _point_adjoint(i2, kind, p, params, gbar):
+    p = np.ascontiguousarray(p, np.float64)
+    params = np.ascontiguousarray(params, np.float64)
+    gbar = np.ascontiguousarray(gbar, np.float64)
+    f = np.zeros(4)
+    pb = np.zeros(3)
+    parb = np.zeros(len(params))
+    P = ctypes.c_void_p
+    st = i2.lib().i2_host_point_adjoint(ctypes.c_int(kind), p.ctypes.data_as(P), params.ctypes.data_as(P),
+                                        gbar.ctypes.data_as(P), f.ctypes.data_as(P), pb.ctypes.data_as(P),
+                                        parb.ctypes.data_as(P))
+    assert st == 0
+    return f, pb, parb
+
+
+def _unit(v):
+    return v / np.linalg.norm(v)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_point_adjoints_match_central_differences(i2, kind):
+    """The hand-derived reverse mode of thetaPsi and of the two singular parts:
+    gbar . df/dx against central differences of the same function for the
+    point and every parameter (1e-7 relative), and the forward value against
+    the kernels' forward functions."""
+    rng = np.random.default_rng(10 + kind)
+    for _ in range(20):
+        A, B, C = rng.normal(size=(3, 3))
+        if kind == 0:
+            params = np.concatenate([A, B, C, _unit(C - B), _unit(A - C), _unit(B - A)])
+            p = (A + B + C) / 3 + rng.normal(size=3)
+        elif kind == 1:    # K_j = (JA, JB, JC) with JB-JC the shared edge, p near the edge on another panel
+            params = np.concatenate([B, C, _unit(A - C), _unit(B - A), _unit(C - B), [1.0 / np.linalg.norm(C - B)]])
+            p = 0.5 * (B + C) + 0.3 * rng.normal(size=3)
+        else:              # shared vertex JA
+            e = _unit(np.cross(rng.normal(size=3), rng.normal(size=3)))
+            params = np.concatenate([A, _unit(A - C), _unit(B - A), e, [0.7]])
+            p = A + 0.5 * rng.normal(size=3)
+        gbar = rng.normal(size=4)
+        f, pb, parb = _point_adjoint(i2, kind, p, params, gbar)
+        if kind == 0:
+            ref = _host_theta_psi(i2, 1, p, params[:9])[0]
+            assert np.abs(f - ref).max() <= 1e-14 * np.abs(ref).max()   # tangents normalised by numpy here
+        h = 1e-6
+        def phi(pp, qq):
+            return _point_adjoint(i2, kind, pp, qq, gbar)[0] @ gbar
+        fd_p = np.array([(phi(p + h * ek, params) - phi(p - h * ek, params)) / (2 * h) for ek in np.eye(3)])
+        fd_q = np.array([(phi(p, params + h * ek) - phi(p, params - h * ek)) / (2 * h) for ek in np.eye(len(params))])
+        scale = max(np.abs(fd_p).max(), np.abs(fd_q).max())
+        assert np.abs(pb - fd_p).max() <= 1e-7 * scale, (pb, fd_p)
+        assert np.abs(parb - fd_q).max() <= 1e-7 * scale, (parb, fd_q)
+
+
+def test_loss_weights_match_oracle(i2, oracle_mod):
+    ij = np.array([[0, 1], [5, 3], [319, 0], [17, 200]], np.int32)
+    out = np.zeros((len(ij), 4))
+    P = ctypes.c_void_p
+    st = i2.lib().i2_host_loss_weights(ctypes.c_uint64(1), ij.ctypes.data_as(P), ctypes.c_int64(len(ij)),
+                                       ctypes.c_int(320), out.ctypes.data_as(P))
+    assert st == 0
+    for q, (i, j) in enumerate(ij):
+        for s in range(4):
+            ref = oracle_mod.normal_draw(1, int(i) * 320 + int(j), s)
+            assert abs(out[q, s] - ref) <= 1e-14 * max(1.0, abs(ref))

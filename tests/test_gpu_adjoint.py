@@ -1,0 +1,144 @@
+"""GPU adjoint dL/dX against the oracle's forward-mode dual numbers and
+against central finite differences (h = 1e-6), both with the forward pass's
+refinement levels frozen.  The finite differences re-evaluate only the pairs
+that touch the moved vertex, so summation noise of the untouched pairs
+cancels, and run in long double: near-degenerate vertex-sharing pairs carry
+~1e-13 of FP64 rounding noise in their closed forms, which a 1e-6 step would
+turn into 1e-7 errors.  The dual-number gate is condition-aware like the
+forward parity gate: 1e-8 plus 8 times the FP64 noise of the oracle's own
+derivative (double vs long double duals).  Also: the loss value against the
+oracle, and a torch.autograd.Function wrapper (test harness only)."""
+import numpy as np
+import pytest
+
+from tests.helpers import reported_levels
+from tests.test_gpu_parity import run_gpu
+
+pytestmark = pytest.mark.gpu
+
+SEED = 1
+
+
+def frozen_levels(out, Ks, quirk, fixed=None):
+    return {t: (np.full(len(out[t]["conv"]), fixed, np.int8) if fixed is not None
+                else reported_levels(out[t]["conv"], Ks[t], quirk)) for t in range(3)}
+
+
+def oracle_loss(orc, i2, out, rl):
+    return sum(orc.loss_tasks(t, out[t]["tasks"], rl[t], i2.QF3D13_L12, i2.QF3D13_W, 7, SEED) for t in range(3)
+               if len(out[t]["tasks"]))
+
+
+def oracle_derivatives(orc, i2, tri, out, rl, coords, h=1e-6):
+    """(long-double dual, FP64 dual, long-double central difference) of L along
+    each (vertex, axis) in coords."""
+    dual_ld, dual, fd = [], [], []
+    for vtx, axis in coords:
+        dl = d = f = 0.0
+        for t in range(3):
+            tasks = out[t]["tasks"]
+            if not len(tasks):
+                continue
+            touch = (tri[tasks[:, 0]] == vtx).any(axis=1) | (tri[tasks[:, 1]] == vtx).any(axis=1)
+            if not touch.any():
+                continue
+            args = (t, tasks[touch], rl[t][touch], i2.QF3D13_L12, i2.QF3D13_W, 7, SEED)
+            dl += orc.loss_tasks(*args, vtx=vtx, axis=axis, dual=True, long_double=True)
+            d += orc.loss_tasks(*args, vtx=vtx, axis=axis, dual=True)
+            f += (orc.loss_tasks(*args, vtx=vtx, axis=axis, h=h, long_double=True) -
+                  orc.loss_tasks(*args, vtx=vtx, axis=axis, h=-h, long_double=True)) / (2 * h)
+        dual_ld.append(dl)
+        dual.append(d)
+        fd.append(f)
+    return np.array(dual_ld), np.array(dual), np.array(fd)
+
+
+def check_adjoint(gpu, oracle_mod, xyz, tri, label, ncoords=64, **run):
+    objs, out, Ks = run_gpu(gpu, xyz, tri, check=False, **run)
+    ev = objs[2]
+    L, G = ev.lossAndGrad(SEED)
+    print(f"{label}: K {Ks}, adjoint {ev.adjointMs():.2f} ms", flush=True)
+    orc = oracle_mod.Oracle(xyz, tri)
+    rl = frozen_levels(out, Ks, run.get("quirk", True), run.get("fixed"))
+    L_orc = oracle_loss(orc, gpu, out, rl)
+    assert abs(L - L_orc) <= 1e-10 * max(abs(L_orc), 1.0), (L, L_orc)
+    rng = np.random.default_rng(SEED)
+    coords = [(int(rng.integers(len(xyz))), int(rng.integers(3))) for _ in range(ncoords)]
+    dual_ld, dual, fd = oracle_derivatives(orc, gpu, tri, out, rl, coords)
+    g = np.array([G[v, a] for v, a in coords])
+    floor = 1e-2 * np.abs(dual_ld).max()
+    scale = np.maximum(np.abs(dual_ld), floor)
+    e_dual = np.abs(g - dual_ld) / scale
+    nu = np.abs(dual - dual_ld) / scale
+    e_fd = np.abs(g - fd) / np.maximum(np.abs(fd), floor)
+    ratio = e_dual / (1e-8 + 8 * nu)
+    print(f"{label}: L {L:.6e} (oracle {L_orc:.6e}), |g| max {np.abs(g).max():.3e}, "
+          f"max rel err vs dual {e_dual.max():.2e} (oracle FP64 noise up to {nu.max():.1e}, worst err/gate {ratio.max():.3f}), "
+          f"vs central differences {e_fd.max():.2e}", flush=True)
+    assert (ratio <= 1.0).all(), ratio.max()
+    assert e_fd.max() <= 1e-6, e_fd.max()
+    return ev, G
+
+
+def sphere(gpu, level=2, seed=0):
+    return gpu.icosphere(level, (1.0, 1.0, 1.0), 0.05, seed)
+
+
+@pytest.mark.parametrize("path", [0, 1])
+def test_adjoint_auto_refinement(gpu, oracle_mod, path):
+    xyz, tri = sphere(gpu)
+    check_adjoint(gpu, oracle_mod, xyz, tri, f"sphere-2 auto path {path}", maxLevel=3, tol=1e-8, quirk=True, path=path)
+
+
+def test_adjoint_without_buffer_swap(gpu, oracle_mod):
+    xyz, tri = sphere(gpu, seed=4)
+    check_adjoint(gpu, oracle_mod, xyz, tri, "sphere-2 auto, no swap", ncoords=24, maxLevel=3, tol=1e-8, quirk=False, path=1)
+
+
+@pytest.mark.parametrize("level", [0, 2])
+def test_adjoint_fixed_levels(gpu, oracle_mod, level):
+    """Level 2 sends every disjoint pair through the deep-pair queue (and its
+    growth after an overflow)."""
+    xyz, tri = sphere(gpu, seed=2)
+    check_adjoint(gpu, oracle_mod, xyz, tri, f"sphere-2 fixed {level}", ncoords=16, fixed=level, path=0)
+
+
+def test_adjoint_ellipsoid_two_paths_agree(gpu):
+    xyz, tri = gpu.icosphere(2, (2.0, 1.0, 0.6), 0.05, 5)
+    grads = []
+    for path in (0, 1):
+        objs, _, _ = run_gpu(gpu, xyz, tri, maxLevel=3, tol=1e-8, quirk=True, path=path, check=False)
+        grads.append(objs[2].lossAndGrad(SEED))
+    (L0, G0), (L1, G1) = grads
+    assert abs(L0 - L1) <= 1e-10 * abs(L0)
+    assert np.abs(G0 - G1).max() <= 1e-9 * np.abs(G0).max()
+
+
+def test_autograd_function_wrapper(gpu):
+    """dL/dX through torch.autograd.Function: forward runs the integrator, backward
+    hands the adjoint kernels' gradient to autograd."""
+    import torch
+
+    class PairLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, X, tri):
+            xyz = X.detach().cpu().numpy()
+            objs, _, _ = run_gpu(gpu, xyz, tri, maxLevel=3, tol=1e-8, quirk=True, path=1, check=False)
+            L, G = objs[2].lossAndGrad(SEED)
+            ctx.save_for_backward(torch.from_numpy(G))
+            return torch.tensor(L, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gL):
+            (G,) = ctx.saved_tensors
+            return gL * G, None
+
+    xyz, tri = sphere(gpu, level=1)
+    X = torch.tensor(xyz, dtype=torch.float64, requires_grad=True)
+    L = PairLoss.apply(X, tri)
+    (3.0 * L).backward()
+    objs, _, _ = run_gpu(gpu, xyz, tri, maxLevel=3, tol=1e-8, quirk=True, path=1, check=False)
+    L_ref, G_ref = objs[2].lossAndGrad(SEED)
+    # atomics make the summation order (not the terms) run-dependent
+    assert abs(float(L) - L_ref) <= 1e-12 * abs(L_ref)
+    assert np.allclose(X.grad.numpy(), 3.0 * G_ref, rtol=1e-10, atol=1e-12 * np.abs(G_ref).max())
