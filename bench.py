@@ -15,6 +15,12 @@ sweep, refinement levels 0 and 1), timed with HIP events on the library's
 stream.  `cpu_baseline` times the CPU oracle (FP64 OpenMP restatement of the
 reference) on a stratified sample of the same tasks.
 
+`autotune` is the batch autotuner's table for this workload (pairs per
+workgroup x points per lane, median ms of graph replays); the timed steps use
+its choice.  `batch64` is 64 ragged meshes (icosphere levels U{2,3,4,5},
+semi-axes U(0.5, 3.0), about 7e9 ordered pairs) in one captured graph with
+row-sum outputs, tuned by the autotuner on its first 8 meshes.
+
 `adjoint` is the shape-sensitivity step on the jittered unit sphere of the
 same size (icosphere-4, axes (1, 1, 1), same jitter and refinement): the
 forward pass, then the loss L = sum w Theta + v . J over all ordered pairs
@@ -70,6 +76,41 @@ def time_steps(ev, steps, warmup):
         ev.runAllPairs(False)
     i2.synchronize(0)
     return (time.perf_counter() - t0) / steps
+
+
+def batch64_step():
+    """64 ragged meshes in one graph (row sums), autotuned on the first 8."""
+    levels, axes = i2.batch_spec(64, 0)
+    meshes = []
+    for m in range(64):
+        x, t = i2.icosphere(int(levels[m]), tuple(axes[m]), 0.05, 1000 + m)
+        mesh = i2.Mesh3D(0)
+        mesh.setData(x, t)
+        mesh.prepareMesh()
+        meshes.append(mesh)
+    integ = i2.NumericalIntegrator3D(meshes[0])
+    integ.setErrorControl(3, 1e-8, True)
+    sub = i2.Batch(0)
+    for mesh in meshes[:8]:
+        sub.addMesh(mesh)
+    sub.configure(integ, i2.BATCH_ROW_SUMS)
+    P, ppl, grid = sub.autotune(reps=1)
+    del sub
+    b = i2.Batch(0)
+    for mesh in meshes:
+        b.addMesh(mesh)
+    b.configure(integ, i2.BATCH_ROW_SUMS)
+    b.setTuning(P, ppl)
+    b.run()
+    t0 = time.perf_counter()
+    ms = b.run()
+    wall = time.perf_counter() - t0
+    n = b.taskCount()
+    return {"workload": "64 icospheres, levels U{2,3,4,5} (seed 0): " + ",".join(str(int(x)) for x in levels) +
+                        "; semi-axes U(0.5, 3.0); jitter 0.05 mean edge (seed 1000 + mesh); refinement up to 3 "
+                        "levels at tol 1e-8; row-sum outputs",
+            "ordered_pairs": int(n), "ms": ms, "wall_ms": 1e3 * wall, "value": n / wall, "unit": "pair-integrals/s",
+            "tuning": [P, ppl], "autotune_table_ms": {f"{k[0]}x{k[1]}": round(v, 2) for k, v in grid.items()}}
 
 
 def adjoint_step(args):
@@ -129,6 +170,8 @@ def main():
     ap.add_argument("--pairs-per-workgroup", type=int, default=64)
     ap.add_argument("--points-per-lane", type=int, default=13)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-batch64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1"}))
@@ -139,6 +182,16 @@ def main():
     xyz, tri, mesh = make_mesh()
     integ = i2.NumericalIntegrator3D(mesh)
     integ.setErrorControl(3, 1e-8, True)
+
+    tune = None
+    if not args.no_autotune:
+        tb = i2.Batch(0)
+        tb.addMesh(mesh)
+        tb.configure(integ, i2.BATCH_PER_PAIR)
+        P, ppl, grid = tb.autotune(reps=3)
+        args.pairs_per_workgroup, args.points_per_lane = P, ppl
+        tune = {"choice": [P, ppl], "table_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in grid.items()}}
+        del tb
 
     fast = i2.EvaluatorJ3DK(mesh, integ)
     fast.setPath(1)
@@ -191,7 +244,10 @@ def main():
                      "flops_per_point": flops_per_point(), "points": points,
                      "algorithmic_bytes": traffic_bytes},
     }
+    out["autotune"] = tune
     out["adjoint"] = adjoint_step(args)
+    if not args.no_batch64:
+        out["batch64"] = batch64_step()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(xyz, tri)
     print(json.dumps(out))

@@ -195,6 +195,12 @@ int i2_batch_set_tuning(i2_batch* b, int pairsPerWorkgroup, int pointsPerLane);
 /* Runs every task of every mesh; the first run after a change captures the
  * graph, later runs replay it.  ms receives the device time of the run. */
 int i2_batch_run(i2_batch* b, double* ms);
+/* Autotuner (no counterpart in the reference): times the batch's graph for
+ * pairs per workgroup {16, 32, 64, 128} x points per lane {1, 2, 4} (and 13,
+ * the whole rule per lane, when includeWholeRule), the median of `reps`
+ * replays each, and leaves the batch on the fastest.  table_ms receives the
+ * medians, row-major over (pairs per workgroup, points per lane). */
+int i2_batch_autotune(i2_batch* b, int reps, int includeWholeRule, int* bestP, int* bestPPL, double* table_ms);
 int i2_batch_get_pair_results(const i2_batch* b, int mesh, int type, double* integrals4, double* results3, int8_t* conv);
 /* rows4[i] = (sum_j J_ij, sum_j Theta_ij) over all three types. */
 int i2_batch_get_row_sums(const i2_batch* b, int mesh, double* rows4);

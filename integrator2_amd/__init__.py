@@ -342,6 +342,17 @@ class Batch:
         _check(lib().i2_batch_run(self._h, ctypes.byref(ms)))
         return ms.value
 
+    def autotune(self, reps=2, includeWholeRule=True):
+        """Times pairs-per-workgroup {16, 32, 64, 128} x points-per-lane {1, 2, 4} (+ 13) and keeps
+        the fastest; returns (pairsPerWorkgroup, pointsPerLane, {(P, ppl): median ms})."""
+        ppls = [1, 2, 4, 13] if includeWholeRule else [1, 2, 4]
+        table = np.zeros(4 * len(ppls))
+        bp, bl = ctypes.c_int(), ctypes.c_int()
+        _check(lib().i2_batch_autotune(self._h, ctypes.c_int(reps), ctypes.c_int(int(includeWholeRule)),
+                                       ctypes.byref(bp), ctypes.byref(bl), _ptr(table)))
+        grid = {(P, l): float(table[a * len(ppls) + k]) for a, P in enumerate([16, 32, 64, 128]) for k, l in enumerate(ppls)}
+        return bp.value, bl.value, grid
+
     def setProfiling(self, enable=True):
         _check(lib().i2_batch_set_profiling(self._h, ctypes.c_int(int(enable))))
 

@@ -990,6 +990,32 @@ int i2_batch_run(i2_batch* b, double* ms) {
     return batch_run(b, ms);
 }
 
+int i2_batch_autotune(i2_batch* b, int reps, int includeWholeRule, int* bestP, int* bestPPL, double* table_ms) {
+    if (!b || reps < 1) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (b->profile) { set_error("autotuning times graph replays: turn profiling off first"); return I2_ERR_STATE; }
+    static const int Ps[4] = {16, 32, 64, 128};
+    static const int PPLs[4] = {1, 2, 4, 13};
+    const int nppl = includeWholeRule ? 4 : 3;
+    double best = 1e300;
+    int bp = b->P, bl = b->ppl;
+    std::vector<double> t((size_t)reps);
+    for (int a = 0; a < 4; ++a)
+        for (int l = 0; l < nppl; ++l) {
+            I2_TRY(i2_batch_set_tuning(b, Ps[a], PPLs[l]));
+            double ms = 0.0;
+            I2_TRY(batch_run(b, &ms));   // the first run after a change captures the graph
+            for (int r = 0; r < reps; ++r) I2_TRY(batch_run(b, &t[(size_t)r]));
+            std::sort(t.begin(), t.end());
+            const double med = t[(size_t)reps / 2];
+            if (table_ms) table_ms[a * nppl + l] = med;
+            if (med < best) { best = med; bp = Ps[a]; bl = PPLs[l]; }
+        }
+    I2_TRY(i2_batch_set_tuning(b, bp, bl));
+    if (bestP) *bestP = bp;
+    if (bestPPL) *bestPPL = bl;
+    return I2_OK;
+}
+
 int i2_batch_get_pair_results(const i2_batch* b, int mesh, int type, double* integrals4, double* results3, int8_t* conv) {
     if (!b || mesh < 0 || mesh >= (int)b->meshes.size() || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
     if (!b->ran || b->out != OUT_PER_PAIR) { set_error("no per-pair results"); return I2_ERR_STATE; }

@@ -134,6 +134,18 @@ class Oracle:
         d, ld = self.singular_integrals(t, ij)
         return np.abs(d - ld).max(axis=1) / np.maximum(np.abs(np.asarray(totals)).max(axis=1), 1e-300)
 
+    def values_at_levels(self, t, ij, rlevel, L12, w, order):
+        """Integrals (n,4) and J (n,3) of ordered tasks at the given reported levels."""
+        ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
+        rlevel = np.ascontiguousarray(rlevel, np.int8)
+        L12 = np.ascontiguousarray(L12, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        I4 = np.zeros((len(ij), 4))
+        J3 = np.zeros((len(ij), 3))
+        lib().o_tasks_at_levels(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12), _ptr(w),
+                                ctypes.c_int(len(w)), ctypes.c_int(order), _ptr(I4), _ptr(J3))
+        return I4, J3
+
     def level_value(self, t, i, j, L12, w, order, level):
         L12 = np.ascontiguousarray(L12, np.float64)
         w = np.ascontiguousarray(w, np.float64)

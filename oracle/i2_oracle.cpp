@@ -453,6 +453,24 @@ double o_loss_tasks(void* h, int type, const int32_t* ij, const int8_t* rlevel, 
     return o_loss_tasks_prec(h, type, ij, rlevel, n, L12, w, nPts, order, seed, vtx, axis, hstep, dual, 0, nthreads);
 }
 
+// Integrals (numerical part at level rlevel[q] plus the closed singular part)
+// and J of ordered tasks at given levels, for comparing values when the
+// refinement decisions of the compared runs are imposed.
+void o_tasks_at_levels(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n, const double* L12,
+                       const double* w, int nPts, int order, double* out4, double* out3) {
+    const Mesh& m = *(Mesh*)h;
+    Rule R = make_rule(L12, w, nPts, order);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < n; ++q) {
+        const int i = ij[2 * q], j = ij[2 * q + 1];
+        TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
+        const Q4<double> v = task_level_value(m, R, type, i, j, rlevel[q], g) + singular_integral(m, type, i, j, g);
+        const V3<double> J = (type == VERTEX_SHARING) ? finalize_vertex(v, g.nJ, g.areaI) : finalize_plain(v, g.nJ);
+        out4[4 * q] = v.x; out4[4 * q + 1] = v.y; out4[4 * q + 2] = v.z; out4[4 * q + 3] = v.w;
+        out3[3 * q] = J.x; out3[3 * q + 1] = J.y; out3[3 * q + 2] = J.z;
+    }
+}
+
 // Wall-clock helper for the CPU baseline.
 double o_now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();

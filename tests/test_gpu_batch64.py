@@ -1,0 +1,124 @@
+"""The 64-mesh batch: icospheres of subdivision level U{2,3,4,5} mapped to
+ellipsoids with semi-axes U(0.5, 3.0) (seed 0), normal jitter 0.05 of the mean
+edge (seed 1000 + mesh), automatic refinement up to 3 levels at tolerance
+1e-8, all meshes in one captured graph, row sums per control panel
+(sum over j of J and Theta over all three types).  About 7e9 ordered pairs.
+
+Checks: every level-2 mesh against the oracle's per-pair values summed per
+row, and K per (mesh, type); one level-3 and one level-4 mesh against a
+single-mesh per-pair run of the batched path summed on the host (the
+cross-mesh item and queue indexing); the autotuner's choice is one of its grid."""
+import numpy as np
+import pytest
+
+from tests.helpers import TOL, check_decisions, reported_levels
+
+pytestmark = pytest.mark.gpu
+
+
+def batch_meshes(i2, n=64):
+    levels, axes = i2.batch_spec(n, 0)
+    out = []
+    for m in range(n):
+        xyz, tri = i2.icosphere(int(levels[m]), tuple(axes[m]), 0.05, 1000 + m)
+        out.append((int(levels[m]), xyz, tri))
+    return out
+
+
+def add_rows(rows, absrows, i, vals):
+    for c in range(4):
+        rows[:, c] += np.bincount(i, weights=vals[:, c], minlength=len(rows))
+    absrows += np.bincount(i, weights=np.abs(vals).max(axis=1), minlength=len(rows))
+
+
+def oracle_rows(orc, i2, nT, single, label, maxLevel=3, tol=1e-8):
+    """Oracle row sums at the GPU's reported levels (from a per-pair run of the
+    same mesh), after checking the GPU's refinement decisions against the
+    oracle's own; K per type."""
+    rows = np.zeros((nT, 4))
+    absrows = np.zeros(nT)
+    Ks = []
+    for t in range(3):
+        tasks = orc.ordered_tasks(t)
+        I4, J3, conv, K, lev = orc.run_tasks(t, tasks, i2.QF3D13_L12, i2.QF3D13_W, 7, mode=1, level=maxLevel, tol=tol,
+                                             want_levels=True)
+        Ks.append(K)
+        conv_gpu = single.pairResults(0, t)["conv"]
+        check_decisions(conv_gpu, conv, lev, tol, f"{label} type {t}")
+        rl = reported_levels(conv_gpu, single.refinementIterations(0, t), True)
+        I4g, J3g = orc.values_at_levels(t, tasks, rl, i2.QF3D13_L12, i2.QF3D13_W, 7)
+        add_rows(rows, absrows, tasks[:, 0], np.concatenate([J3g, I4g[:, 3:4]], axis=1))
+    return rows, absrows, Ks
+
+
+def test_config3_batch_row_sums(gpu, oracle_mod):
+    specs = batch_meshes(gpu)
+    meshes = []
+    b = gpu.Batch(0)
+    for lev, xyz, tri in specs:
+        m = gpu.Mesh3D(0)
+        m.setData(xyz, tri)
+        m.prepareMesh()
+        meshes.append(m)
+        b.addMesh(m)
+    integ = gpu.NumericalIntegrator3D(meshes[0])
+    integ.setErrorControl(3, 1e-8, True)
+    b.configure(integ, gpu.BATCH_ROW_SUMS)
+    total = b.taskCount()
+    ms = b.run()
+    print(f"64 meshes, {total} ordered pairs, {ms:.1f} ms, {total / ms * 1e3:.3e} pairs/s", flush=True)
+    assert total == sum(len(t) * (len(t) - 1) for _, _, t in specs)
+    checked = 0
+    for k, (lev, xyz, tri) in enumerate(specs):
+        if lev != 2:
+            continue
+        orc = oracle_mod.Oracle(xyz, tri)
+        single = gpu.Batch(0)
+        single.addMesh(meshes[k])
+        single.configure(integ, gpu.BATCH_PER_PAIR)
+        single.run()
+        rows, absrows, Ks = oracle_rows(orc, gpu, len(tri), single, f"mesh {k}")
+        got = b.rowSums(k)
+        err = np.abs(got - rows).max(axis=1) / absrows
+        assert err.max() <= 10 * TOL, (k, err.max())
+        assert [b.refinementIterations(k, t) for t in range(3)] == Ks, k
+        checked += 1
+    assert checked >= 8
+    # larger meshes: single-mesh per-pair runs of the batched path, summed per row
+    for target in (3, 4):
+        k = next(k for k, (lev, _, _) in enumerate(specs) if lev == target)
+        one = gpu.Batch(0)
+        one.addMesh(meshes[k])
+        one.configure(integ, gpu.BATCH_PER_PAIR)
+        one.run()
+        orc = oracle_mod.Oracle(specs[k][1], specs[k][2])
+        nT = len(specs[k][2])
+        rows, absrows = np.zeros((nT, 4)), np.zeros(nT)
+        for t in range(3):
+            tasks = orc.ordered_tasks(t)
+            r = one.pairResults(0, t)
+            add_rows(rows, absrows, tasks[:, 0], np.concatenate([r["results"], r["integrals"][:, 3:4]], axis=1))
+        got = b.rowSums(k)
+        err = np.abs(got - rows).max(axis=1) / absrows
+        print(f"level-{target} mesh {k}: max row error {err.max():.2e} (summation order only)", flush=True)
+        assert err.max() <= 1e-13, (target, err.max())
+
+
+def test_autotuner_picks_from_its_grid(gpu):
+    b = gpu.Batch(0)
+    meshes = []
+    for lev, xyz, tri in batch_meshes(gpu, 6):
+        m = gpu.Mesh3D(0)
+        m.setData(xyz, tri)
+        m.prepareMesh()
+        meshes.append(m)
+        b.addMesh(m)
+    integ = gpu.NumericalIntegrator3D(meshes[0])
+    integ.setErrorControl(3, 1e-8, True)
+    b.configure(integ, gpu.BATCH_ROW_SUMS)
+    P, ppl, grid = b.autotune(reps=1)
+    print("autotune:", P, ppl, {k: round(v, 2) for k, v in grid.items()}, flush=True)
+    assert (P, ppl) in grid and grid[(P, ppl)] == min(grid.values())
+    assert len(grid) == 16 and all(v > 0 for v in grid.values())
+    ms = b.run()
+    assert ms > 0
