@@ -19,7 +19,7 @@ reference) on a stratified sample of the same tasks.
 workgroup x points per lane, median ms of graph replays); the timed steps use
 its choice.  `batch64` is 64 ragged meshes (icosphere levels U{2,3,4,5},
 semi-axes U(0.5, 3.0), about 7e9 ordered pairs) in one captured graph with
-row-sum outputs, tuned by the autotuner on its first 8 meshes.
+row-sum outputs, tuned by the autotuner on its first 4 meshes.
 
 `adjoint` is the shape-sensitivity step on the jittered unit sphere of the
 same size (icosphere-4, axes (1, 1, 1), same jitter and refinement): the
@@ -78,8 +78,12 @@ def time_steps(ev, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def batch64_step():
-    """64 ragged meshes in one graph (row sums), autotuned on the first 8."""
+    """64 ragged meshes in one graph (row sums), autotuned on the first 4."""
     levels, axes = i2.batch_spec(64, 0)
     meshes = []
     for m in range(64):
@@ -91,11 +95,13 @@ def batch64_step():
     integ = i2.NumericalIntegrator3D(meshes[0])
     integ.setErrorControl(3, 1e-8, True)
     sub = i2.Batch(0)
-    for mesh in meshes[:8]:
+    for mesh in meshes[:4]:
         sub.addMesh(mesh)
     sub.configure(integ, i2.BATCH_ROW_SUMS)
+    progress("batch64: autotuning on 4 meshes")
     P, ppl, grid = sub.autotune(reps=1)
     del sub
+    progress(f"batch64: tuned {P}x{ppl}; running 64 meshes")
     b = i2.Batch(0)
     for mesh in meshes:
         b.addMesh(mesh)
@@ -184,6 +190,7 @@ def main():
     integ.setErrorControl(3, 1e-8, True)
 
     tune = None
+    progress("config 2: autotuning")
     if not args.no_autotune:
         tb = i2.Batch(0)
         tb.addMesh(mesh)
@@ -196,7 +203,9 @@ def main():
     fast = i2.EvaluatorJ3DK(mesh, integ)
     fast.setPath(1)
     fast.setTuning(args.pairs_per_workgroup, args.points_per_lane)
+    progress("config 2: batched path")
     t_fast = time_steps(fast, args.steps, args.warmup)
+    progress("config 2: plain path")
 
     plain = i2.EvaluatorJ3DK(mesh, integ)
     plain.setPath(0)
@@ -245,10 +254,12 @@ def main():
                      "algorithmic_bytes": traffic_bytes},
     }
     out["autotune"] = tune
+    progress("adjoint")
     out["adjoint"] = adjoint_step(args)
     if not args.no_batch64:
         out["batch64"] = batch64_step()
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(xyz, tri)
     print(json.dumps(out))
 

@@ -52,8 +52,10 @@ int i2_device_count(int* n);
  * reference's expression (thetaPsi, src/evaluators/evaluatorJ3DK.cu:266-313),
  * fast = 1 the division-free form used by the batched kernels.  For checking. */
 int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9, double* out4);
-/* Host-side evaluation of the kernels' FP64 logarithm (fn = 0, of x) and
- * two-argument arctangent (fn = 1, atan2(y, x)).  For checking. */
+/* Host-side evaluation of the kernels' FP64 logarithm (fn = 0, of x),
+ * two-argument arctangent (fn = 1, atan2(y, x)) and square root (fn = 2, of
+ * x; the host uses the library's 1/sqrt as the estimate the device takes
+ * from the hardware).  For checking. */
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out);
 
 /* ======================================================================
@@ -195,6 +197,13 @@ int i2_batch_set_tuning(i2_batch* b, int pairsPerWorkgroup, int pointsPerLane);
 /* Runs every task of every mesh; the first run after a change captures the
  * graph, later runs replay it.  ms receives the device time of the run. */
 int i2_batch_run(i2_batch* b, double* ms);
+/* Appends the adjoint of the loss of i2_evaluator_loss_and_grad (weights from
+ * `seed`) to the batch's graph: one replay runs the forward pass and dL/dX
+ * of every mesh, with per-pair terms recomputed, not stored (row-sum mode
+ * keeps one convergence byte per pair).  No counterpart in the reference. */
+int i2_batch_enable_adjoint(i2_batch* b, int enable, uint64_t seed);
+/* Loss and dL/dX (3*nV doubles) of one mesh of the last run. */
+int i2_batch_get_loss_and_grad(const i2_batch* b, int mesh, double* loss, double* dLdX);
 /* Autotuner (no counterpart in the reference): times the batch's graph for
  * pairs per workgroup {16, 32, 64, 128} x points per lane {1, 2, 4} (and 13,
  * the whole rule per lane, when includeWholeRule), the median of `reps`

@@ -342,6 +342,17 @@ class Batch:
         _check(lib().i2_batch_run(self._h, ctypes.byref(ms)))
         return ms.value
 
+    def enableAdjoint(self, enable=True, seed=1):
+        """Append the adjoint of L = sum w Theta + v . J (weights from seed) to the batch graph."""
+        _check(lib().i2_batch_enable_adjoint(self._h, ctypes.c_int(int(enable)), ctypes.c_uint64(seed)))
+
+    def lossAndGrad(self, mesh):
+        nV = self.meshes[mesh].sizes()[0]
+        grad = np.zeros((nV, 3))
+        loss = ctypes.c_double()
+        _check(lib().i2_batch_get_loss_and_grad(self._h, ctypes.c_int(mesh), ctypes.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
     def autotune(self, reps=2, includeWholeRule=True):
         """Times pairs-per-workgroup {16, 32, 64, 128} x points-per-lane {1, 2, 4} (+ 13) and keeps
         the fastest; returns (pairsPerWorkgroup, pointsPerLane, {(P, ppl): median ms})."""

@@ -102,11 +102,15 @@ __device__ __forceinline__ void add3(double* dst, d3 v) {
 }
 __device__ __forceinline__ double dot4(q4 a, q4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
 
-struct RuleSh { double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS]; };
+struct RuleSh {
+    double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
+    i2m::LogEntry lt[i2m::LOGTAB_N];
+};
 __device__ __forceinline__ void load_rule_sh(RuleSh& s, const Rule& R) {
     for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
         s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
     }
+    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) s.lt[k] = i2m::c_logtab[k];
 }
 
 // Vertex cotangents of a panel from those of its tangents and normal
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_disjoint(AdjArgs A) {
                     p = p + R.L2[k] * b;
                     p = p + R.L3[k] * c;
                     d3 pb;
-                    const q4 f = theta_psi_fast_rev(p, JG, (fa * R.w[k]) * gb, &pb, g);
+                    const q4 f = theta_psi_fast_rev(p, JG, R.lt, (fa * R.w[k]) * gb, &pb, g);
                     F = F + R.w[k] * f;
                     axpy(ab, R.L1[k], pb); axpy(bbar, R.L2[k], pb); axpy(cb, R.L3[k], pb);
                 }
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_deep(AdjArgs A) {
             p = p + R.L2[k] * b;
             p = p + R.L3[k] * c;
             d3 pb;
-            const q4 f = theta_psi_fast_rev(p, JG, (fa * R.w[k]) * gb, &pb, g);
+            const q4 f = theta_psi_fast_rev(p, JG, R.lt, (fa * R.w[k]) * gb, &pb, g);
             F = F + R.w[k] * f;
             const d3 beta = R.L1[k] * ba + R.L2[k] * bb + R.L3[k] * bc;
             axpy(X0, beta.x, pb); axpy(X1, beta.y, pb); axpy(X2, beta.z, pb);
@@ -380,7 +384,7 @@ __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R,
         const q4 gp = (fa * R.w[k]) * gb;
         const q4 gm = (-1.0) * gp;
         d3 pb, ps;
-        q4 f = theta_psi_fast_rev(p, JG, gp, &pb, B.g);
+        q4 f = theta_psi_fast_rev(p, JG, R.lt, gp, &pb, B.g);
         if (TYPE == I2_ATTACHED_NEIGHBORS) f = f - sing_edge_rev(p, ec, gm, &ps, B.ge);
         else f = f - sing_vertex_rev(p, vc, gm, &ps, B.gv);
         pb = pb + ps;
@@ -475,7 +479,7 @@ __global__ void k_adj_zero(double* p, int64_t n, unsigned long long* c, int nc) 
 
 // ------------------------------------------------------------ host side
 int adjoint_launch(AdjState& S, hipStream_t st) {
-    const int64_t nzero = (int64_t)S.d_grad.n;
+    const int64_t nzero = S.grad_off.empty() ? 0 : S.grad_off.back();
     hipLaunchKernelGGL(k_adj_zero, dim3(256), dim3(256), 0, st, S.d_grad.p, nzero, S.d_counters.p, 4);
     I2_TRY(check_launch("k_adj_zero"));
     hipLaunchKernelGGL(k_adj_disjoint, dim3(S.grid), dim3(ADJ_BLOCK), 0, st, S.args);
@@ -489,52 +493,49 @@ int adjoint_launch(AdjState& S, hipStream_t st) {
     return I2_OK;
 }
 
-// Builds the descriptors of one mesh's adjoint from the forward pass's
-// convergence iterations and K per type.
-int adjoint_setup(AdjState& S, int device, const i2_mesh* mesh, const i2_integrator* it, const int8_t* const conv[3],
-                  const int K[3], uint64_t seed) {
+// Builds the adjoint's descriptors for meshes D (panels, lists, convergence
+// iterations and device K already set; dX and loss are assigned here, one
+// block of 3 nV + 1 doubles per mesh).
+int adjoint_setup(AdjState& S, int device, std::vector<AdjMesh> D, const std::vector<int64_t>& nV,
+                  const i2_integrator* it, uint64_t seed) {
     int cus = 0;
     I2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     S.grid = std::max(1, cus) * 4;
-    AdjMesh D{};
-    D.panels = mesh->d_panels.p;
-    D.row_start = mesh->d_row_start.p;
-    D.nbr_start = mesh->d_nbr_start.p;
-    D.nbr = mesh->d_nbr.p;
-    D.lists[0] = mesh->d_simple.p;
-    D.lists[1] = mesh->d_attached.p;
-    for (int t = 0; t < 3; ++t) { D.npairs[t] = mesh->pairs[t]; D.conv[t] = conv[t]; }
-    D.nT = (int)mesh->nT;
-    const int64_t ngrad = 3 * mesh->nV + 1;     // dL/dX, then L
-    I2_TRY(S.d_grad.ensure(ngrad));
-    D.dX = S.d_grad.p;
-    D.loss = S.d_grad.p + 3 * mesh->nV;
-    S.nV = mesh->nV;
-    I2_TRY(S.d_K.ensure(3));
-    I2_HIP(hipMemcpy(S.d_K.p, K, 3 * sizeof(int), hipMemcpyHostToDevice));
-    D.K = S.d_K.p;
-    I2_TRY(S.d_mesh.ensure(1));
-    I2_HIP(hipMemcpy(S.d_mesh.p, &D, sizeof(AdjMesh), hipMemcpyHostToDevice));
+    const int M = (int)D.size();
     AdjArgs& A = S.args;
+    A.j_tile = 32;
+    S.grad_off.assign(M + 1, 0);
+    for (int m = 0; m < M; ++m) S.grad_off[m + 1] = S.grad_off[m] + 3 * nV[m] + 1;
+    I2_TRY(S.d_grad.ensure(S.grad_off[M]));
+    std::vector<int64_t> item_base(M + 1, 0), sing_base(2 * M + 1, 0);
+    for (int m = 0; m < M; ++m) {
+        D[m].dX = S.d_grad.p + S.grad_off[m];
+        D[m].loss = S.d_grad.p + S.grad_off[m] + 3 * nV[m];
+        const int64_t iblocks = (D[m].nT + ADJ_BLOCK - 1) / ADJ_BLOCK;
+        const int64_t jtiles = (D[m].nT + A.j_tile - 1) / A.j_tile;
+        item_base[m + 1] = item_base[m] + iblocks * jtiles;
+    }
+    int64_t acc = 0;
+    for (int kind = 0; kind < 2; ++kind)
+        for (int m = 0; m < M; ++m) { sing_base[kind * M + m] = acc; acc += 2 * D[m].npairs[kind]; }
+    sing_base[2 * M] = acc;
+    S.nV = nV;
+    I2_TRY(S.d_mesh.ensure(M));
+    I2_HIP(hipMemcpy(S.d_mesh.p, D.data(), sizeof(AdjMesh) * M, hipMemcpyHostToDevice));
+    I2_TRY(S.d_item_base.ensure(M + 1));
+    I2_HIP(hipMemcpy(S.d_item_base.p, item_base.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice));
+    I2_TRY(S.d_sing_base.ensure(2 * M + 1));
+    I2_HIP(hipMemcpy(S.d_sing_base.p, sing_base.data(), sizeof(int64_t) * (2 * M + 1), hipMemcpyHostToDevice));
     A.meshes = S.d_mesh.p;
-    A.nmeshes = 1;
+    A.nmeshes = M;
     A.fixed = it->fixed;
     A.last_level = it->fixed ? it->fixed_level : it->max_level;
     A.quirk = it->quirk;
     A.rule = it->rule;
     A.seed = seed;
-    A.j_tile = 32;
-    const int64_t iblocks = (mesh->nT + ADJ_BLOCK - 1) / ADJ_BLOCK;
-    const int64_t jtiles = (mesh->nT + A.j_tile - 1) / A.j_tile;
-    const int64_t h_item_base[2] = {0, iblocks * jtiles};
-    A.disjoint_items = iblocks * jtiles;
-    I2_TRY(S.d_item_base.ensure(2));
-    I2_HIP(hipMemcpy(S.d_item_base.p, h_item_base, sizeof(h_item_base), hipMemcpyHostToDevice));
+    A.disjoint_items = item_base[M];
     A.item_base = S.d_item_base.p;
-    const int64_t h_sing_base[3] = {0, 2 * mesh->pairs[0], 2 * mesh->pairs[0] + 2 * mesh->pairs[1]};
-    A.singular_tasks = h_sing_base[2];
-    I2_TRY(S.d_sing_base.ensure(3));
-    I2_HIP(hipMemcpy(S.d_sing_base.p, h_sing_base, sizeof(h_sing_base), hipMemcpyHostToDevice));
+    A.singular_tasks = sing_base[2 * M];
     A.sing_base = S.d_sing_base.p;
     I2_TRY(S.d_counters.ensure(4));
     A.counters = S.d_counters.p;
@@ -545,27 +546,58 @@ int adjoint_setup(AdjState& S, int device, const i2_mesh* mesh, const i2_integra
     return I2_OK;
 }
 
-// Runs the adjoint, growing the deep queue after an overflow.
+AdjMesh adjoint_mesh(const i2_mesh* mesh, const int8_t* const conv[3], const int* dK) {
+    AdjMesh D{};
+    D.panels = mesh->d_panels.p;
+    D.row_start = mesh->d_row_start.p;
+    D.nbr_start = mesh->d_nbr_start.p;
+    D.nbr = mesh->d_nbr.p;
+    D.lists[0] = mesh->d_simple.p;
+    D.lists[1] = mesh->d_attached.p;
+    for (int t = 0; t < 3; ++t) { D.npairs[t] = mesh->pairs[t]; D.conv[t] = conv[t]; }
+    D.K = dK;
+    D.nT = (int)mesh->nT;
+    return D;
+}
+
+// After a run: false when the deep-pair queue overflowed (it is then grown,
+// and the caller must rebuild and run again).
+int adjoint_check(AdjState& S, hipStream_t st, bool* overflow) {
+    unsigned long long cnt = 0;
+    I2_HIP(hipMemcpyAsync(&cnt, S.d_counters.p + 1, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    I2_HIP(hipStreamSynchronize(st));
+    *overflow = cnt > (unsigned long long)S.deep_cap;
+    if (*overflow) {
+        S.deep_cap = (int64_t)(cnt + cnt / 4 + 1);
+        I2_TRY(S.d_deep.ensure(S.deep_cap));
+        S.args.deep = S.d_deep.p;
+        S.args.deep_cap = S.deep_cap;
+    }
+    return I2_OK;
+}
+
+int adjoint_copy_out(const AdjState& S, int mesh, double* loss, double* dLdX) {
+    const double* base = S.d_grad.p + S.grad_off[mesh];
+    const int64_t nV = S.nV[mesh];
+    if (dLdX) I2_HIP(hipMemcpy(dLdX, base, sizeof(double) * 3 * nV, hipMemcpyDeviceToHost));
+    if (loss) I2_HIP(hipMemcpy(loss, base + 3 * nV, sizeof(double), hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+// Runs the adjoint of one mesh, growing the deep queue after an overflow.
 int adjoint_run(AdjState& S, hipStream_t st, double* loss, double* dLdX) {
     for (int attempt = 0; attempt < 3; ++attempt) {
         I2_HIP(hipEventRecord(S.e0, st));
         I2_TRY(adjoint_launch(S, st));
         I2_HIP(hipEventRecord(S.e1, st));
-        unsigned long long cnt = 0;
-        I2_HIP(hipMemcpyAsync(&cnt, S.d_counters.p + 1, sizeof(cnt), hipMemcpyDeviceToHost, st));
-        I2_HIP(hipStreamSynchronize(st));
-        if (cnt <= (unsigned long long)S.deep_cap) {
+        bool overflow = false;
+        I2_TRY(adjoint_check(S, st, &overflow));
+        if (!overflow) {
             float ms = 0.0f;
             I2_HIP(hipEventElapsedTime(&ms, S.e0, S.e1));
             S.ms = ms;
-            if (dLdX) I2_HIP(hipMemcpy(dLdX, S.d_grad.p, sizeof(double) * 3 * S.nV, hipMemcpyDeviceToHost));
-            if (loss) I2_HIP(hipMemcpy(loss, S.d_grad.p + 3 * S.nV, sizeof(double), hipMemcpyDeviceToHost));
-            return I2_OK;
+            return adjoint_copy_out(S, 0, loss, dLdX);
         }
-        S.deep_cap = (int64_t)(cnt + cnt / 4 + 1);
-        I2_TRY(S.d_deep.ensure(S.deep_cap));
-        S.args.deep = S.d_deep.p;
-        S.args.deep_cap = S.deep_cap;
     }
     set_error("adjoint deep-pair queue did not converge");
     return I2_ERR_STATE;
@@ -611,8 +643,11 @@ int i2_evaluator_loss_and_grad(i2_evaluator* ev, uint64_t seed, double* loss, do
             conv[t] = ev->type[t].conv.p;
         }
     }
-    I2_TRY(adjoint_setup(*ev->adj, ev->mesh->device, ev->mesh, ev->integ, conv, K, seed));
-    return adjoint_run(*ev->adj, ev->stream, loss, dLdX);
+    AdjState& S = *ev->adj;
+    I2_TRY(S.d_K.ensure(3));
+    I2_HIP(hipMemcpy(S.d_K.p, K, 3 * sizeof(int), hipMemcpyHostToDevice));
+    I2_TRY(adjoint_setup(S, ev->mesh->device, {adjoint_mesh(ev->mesh, conv, S.d_K.p)}, {ev->mesh->nV}, ev->integ, seed));
+    return adjoint_run(S, ev->stream, loss, dLdX);
 }
 
 int i2_evaluator_adjoint_ms(const i2_evaluator* ev, double* ms) {
@@ -644,7 +679,7 @@ int i2_host_point_adjoint(int kind, const double* p3, const double* params, cons
     if (kind == 0) {
         const JGeom J = {v(0), v(3), v(6), v(9), v(12), v(15)};
         TPBar g = {z3, z3, z3, z3, z3, z3};
-        f = theta_psi_fast_rev(p, J, gb, &pb, g);
+        f = theta_psi_fast_rev(p, J, i2m::h_logtab, gb, &pb, g);
         put(0, g.a); put(3, g.b); put(6, g.c); put(9, g.ta); put(12, g.tb); put(15, g.tc);
     } else if (kind == 1) {
         const EdgeCtx c = {v(0), v(3), v(6), v(9), v(12), params[15]};

@@ -2,6 +2,8 @@
 #pragma once
 #include "i2_batch.hpp"
 
+#include <vector>
+
 namespace i2 {
 
 // One mesh as the adjoint kernels see it: the forward pass's outputs they
@@ -47,7 +49,8 @@ struct AdjArgs {
 struct AdjState {
     AdjArgs args{};
     int grid = 0;
-    int64_t nV = 0;
+    std::vector<int64_t> nV;       // per mesh
+    std::vector<int64_t> grad_off; // per mesh: dL/dX (3 nV) then L, in d_grad
     int64_t deep_cap = 0;
     double ms = 0.0;              // device time of the last adjoint run
     DBuf<AdjMesh> d_mesh;
@@ -60,5 +63,12 @@ struct AdjState {
 };
 
 void adjoint_release(AdjState*& S);
+AdjMesh adjoint_mesh(const i2_mesh* mesh, const int8_t* const conv[3], const int* dK);
+int adjoint_setup(AdjState& S, int device, std::vector<AdjMesh> D, const std::vector<int64_t>& nV,
+                  const i2_integrator* it, uint64_t seed);
+// Enqueues the adjoint kernels (capturable: no host synchronisation, fills by kernels).
+int adjoint_launch(AdjState& S, hipStream_t st);
+int adjoint_check(AdjState& S, hipStream_t st, bool* overflow);
+int adjoint_copy_out(const AdjState& S, int mesh, double* loss, double* dLdX);
 
 }  // namespace i2

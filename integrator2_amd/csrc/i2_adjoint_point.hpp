@@ -18,23 +18,26 @@ I2_HD __forceinline__ void axpy(d3& y, double a, d3 x) { y.x += a * x.x; y.y += 
 // cotangents of the parameters of thetaPsi over K_j: vertices and edge tangents
 struct TPBar { d3 a, b, c, ta, tb, tc; };
 
-I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, q4 gb, d3* pbar, TPBar& g) {
+I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::LogEntry* __restrict__ lt, q4 gb, d3* pbar,
+                                            TPBar& g) {
     // ---- forward, as theta_psi_fast
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
-    const double la = sqrt(dot(da, da)), lb = sqrt(dot(db, db)), lc = sqrt(dot(dc, dc));
+    const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
     double n1 = la + dot(da, J.tc), e1 = lb + dot(db, J.tc);
     double n2 = lb + dot(db, J.ta), e2 = lc + dot(dc, J.ta);
     double n3 = lc + dot(dc, J.tb), e3 = la + dot(da, J.tb);
     const double h = 0.5 * EPS_PSI_THETA2;
     const bool f1 = fabs(e1) < h * lb, f2 = fabs(e2) < h * lc, f3 = fabs(e3) < h * la;
-    if (f1) { n1 = lb; e1 = la; }
-    if (f2) { n2 = lc; e2 = lb; }
-    if (f3) { n3 = la; e3 = lc; }
+    if (__builtin_expect(f1 | f2 | f3, 0)) {
+        if (f1) { n1 = lb; e1 = la; }
+        if (f2) { n2 = lc; e2 = lb; }
+        if (f3) { n3 = la; e3 = lc; }
+    }
     const double e12 = e1 * e2;
     const double inv = 1.0 / (e12 * e3);
-    const double t1 = i2m::log_pos(n1 * (e2 * e3) * inv);
-    const double t2 = i2m::log_pos(n2 * (e1 * e3) * inv);
-    const double t3 = i2m::log_pos(n3 * e12 * inv);
+    const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);
+    const double t2 = i2m::log_tab(n2 * (e1 * e3) * inv, lt);
+    const double t3 = i2m::log_tab(n3 * e12 * inv, lt);
     q4 r;
     r.x = t1 * J.tc.x + t2 * J.ta.x + t3 * J.tb.x;
     r.y = t1 * J.tc.y + t2 * J.ta.y + t3 * J.tb.y;

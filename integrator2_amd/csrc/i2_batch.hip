@@ -26,6 +26,7 @@
 //   k_levels / k_compute_K   K per (mesh, type) from queue membership.
 //   k_finalize_pairs / k_finalize_rows   parity selection, J, outputs.
 #include "i2_batch.hpp"
+#include "i2_adjoint.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -101,13 +102,17 @@ __device__ __noinline__ q4 singular_integral(int type, const Panel& I, const Pan
                                  vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
 }
 
-// Rule held in LDS: (L1, L2, L3, w) per point
-struct RuleLDS { double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS]; };
+// Rule held in LDS: (L1, L2, L3, w) per point, and the log reduction table
+struct RuleLDS {
+    double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
+    i2m::LogEntry lt[i2m::LOGTAB_N];
+};
 
 __device__ __forceinline__ void load_rule(RuleLDS& s, const Rule& R) {
     for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
         s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
     }
+    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) s.lt[k] = i2m::c_logtab[k];
 }
 
 // Partial quadrature sum_k w_k f(r_k) of one sub-panel over points k = s, s + step, ...
@@ -119,7 +124,7 @@ __device__ __forceinline__ q4 quad_part(const RuleLDS& R, int npts, int s, int s
         d3 p = R.L1[k] * a;
         p = p + R.L2[k] * b;
         p = p + R.L3[k] * c;
-        q4 f = theta_psi_fast(p, J);
+        q4 f = theta_psi_fast(p, J, R.lt);
         if (TYPE != I2_NOT_NEIGHBORS) f = f - sg.at(p);
         acc = acc + R.w[k] * f;
     }
@@ -172,6 +177,7 @@ __device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& 
         M.prev[TYPE][z] = make_double4(prev.x, prev.y, prev.z, prev.w);
         M.conv[TYPE][z] = (int8_t)o.c;
     } else {
+        if (M.conv[TYPE]) M.conv[TYPE][z] = (int8_t)o.c;   // kept for the adjoint
         // K even selects I_c for even c, I_{c-1} for odd c; K odd the opposite
         const q4 v0 = ((o.c & 1) == 0) ? cur : prev;
         const q4 v1 = ((o.c & 1) == 1) ? cur : prev;
@@ -713,6 +719,10 @@ struct i2_batch {
     bool ran = false;
     bool profile = false;
     double sweep_ms = 0.0;
+    // adjoint in the same graph (forward + loss gradient)
+    bool adjoint = false;
+    uint64_t adj_seed = 1;
+    i2::AdjState* adj = nullptr;
 };
 
 namespace i2 {
@@ -762,10 +772,31 @@ static int batch_prepare(i2_batch* b) {
             I2_TRY(o.rowout.ensure(4 * mesh->nT));
             D.rowacc = o.rowacc.p;
             D.rowout = o.rowout.p;
+            if (b->adjoint)
+                for (int t = 0; t < 3; ++t) {
+                    I2_TRY(o.conv[t].ensure(2 * mesh->pairs[t]));
+                    D.conv[t] = o.conv[t].p;
+                }
         }
         E.h_meshes[m] = D;
     }
-    return engine_setup(E, b->cfg);
+    I2_TRY(engine_setup(E, b->cfg));
+    if (!b->adjoint) return I2_OK;
+    if (!b->adj) {
+        b->adj = new AdjState;
+        if (hipEventCreate(&b->adj->e0) != hipSuccess || hipEventCreate(&b->adj->e1) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("event creation failed");
+            return I2_ERR_HIP;
+        }
+    }
+    std::vector<AdjMesh> AD(M);
+    std::vector<int64_t> nV(M);
+    for (int m = 0; m < M; ++m) {
+        AD[m] = adjoint_mesh(b->meshes[m], E.h_meshes[m].conv, E.d_K.p + 3 * m);
+        nV[m] = b->meshes[m]->nV;
+    }
+    return adjoint_setup(*b->adj, b->device, AD, nV, &b->cfg, b->adj_seed);
 }
 
 static int batch_enqueue_all(i2_batch* b) {
@@ -775,7 +806,9 @@ static int batch_enqueue_all(i2_batch* b) {
             hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, b->stream, o.rowacc.p, (int64_t)o.rowacc.n);
             I2_TRY(check_launch("k_zero_f64"));
         }
-    return batch_enqueue(E, b->stream);
+    I2_TRY(batch_enqueue(E, b->stream));
+    if (b->adjoint) I2_TRY(adjoint_launch(*b->adj, b->stream));
+    return I2_OK;
 }
 
 static int batch_capture(i2_batch* b) {
@@ -817,6 +850,11 @@ static int batch_run(i2_batch* b, double* ms) {
         I2_HIP(hipEventRecord(b->e1, b->stream));
         bool overflow = false;
         I2_TRY(engine_check_queues(b->E, b->stream, &overflow, &b->counts));
+        if (b->adjoint) {
+            bool adj_overflow = false;
+            I2_TRY(adjoint_check(*b->adj, b->stream, &adj_overflow));
+            overflow = overflow || adj_overflow;
+        }
         if (!overflow) {
             float f = 0.f;
             I2_HIP(hipEventElapsedTime(&f, b->e0, b->e1));
@@ -933,6 +971,7 @@ void i2_batch_destroy(i2_batch* b) {
         if (b->stream) (void)hipStreamSynchronize(b->stream);
         drop_graph(b);
         b->outs.clear();
+        adjoint_release(b->adj);
         if (b->E.prof0) (void)hipEventDestroy(b->E.prof0);
         if (b->E.prof1) (void)hipEventDestroy(b->E.prof1);
         if (b->e0) (void)hipEventDestroy(b->e0);
@@ -988,6 +1027,23 @@ int i2_batch_set_tuning(i2_batch* b, int pairsPerWorkgroup, int pointsPerLane) {
 int i2_batch_run(i2_batch* b, double* ms) {
     if (!b) { set_error("null batch"); return I2_ERR_INVALID; }
     return batch_run(b, ms);
+}
+
+int i2_batch_enable_adjoint(i2_batch* b, int enable, uint64_t seed) {
+    if (!b) { set_error("null batch"); return I2_ERR_INVALID; }
+    b->adjoint = enable != 0;
+    b->adj_seed = seed;
+    b->dirty = true;
+    b->ran = false;
+    return I2_OK;
+}
+
+int i2_batch_get_loss_and_grad(const i2_batch* b, int mesh, double* loss, double* dLdX) {
+    if (!b || mesh < 0 || mesh >= (int)b->meshes.size() || !loss || !dLdX) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!b->ran || !b->adjoint || !b->adj) { set_error("no adjoint results: enable the adjoint and run the batch"); return I2_ERR_STATE; }
+    DeviceGuard g(b->device);
+    if (!g.ok) return I2_ERR_HIP;
+    return adjoint_copy_out(*b->adj, mesh, loss, dLdX);
 }
 
 int i2_batch_autotune(i2_batch* b, int reps, int includeWholeRule, int* bestP, int* bestPPL, double* table_ms) {

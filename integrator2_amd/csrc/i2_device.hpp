@@ -130,25 +130,29 @@ I2_HD __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, 
 // and the solid-angle atan2 has both arguments multiplied by la lb lc > 0
 // (Van Oosterom and Strackee, IEEE Trans. Biomed. Eng. 30 (1983) 125-126):
 // identical in exact arithmetic, with 3 square roots, 1 division, 3 logs and
-// 1 atan2 per point instead of 13 divisions (log and atan2 from i2_math.hpp,
-// within 2 ulp).  The degenerate-branch tests
-// |1 + (d/l).tau| < eps become |l + d.tau| < eps l.
+// 1 atan2 per point instead of 13 divisions (sqrt, log and atan2 from
+// i2_math.hpp, within 2 ulp; `lt` is the log reduction table, in LDS on the
+// device).  The degenerate-branch tests |1 + (d/l).tau| < eps become
+// |l + d.tau| < eps l.
 struct JGeom { d3 a, b, c, ta, tb, tc; };
-I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J) {
+I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogEntry* __restrict__ lt) {
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
-    const double la = sqrt(dot(da, da)), lb = sqrt(dot(db, db)), lc = sqrt(dot(dc, dc));
+    const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
     double n1 = la + dot(da, J.tc), e1 = lb + dot(db, J.tc);
     double n2 = lb + dot(db, J.ta), e2 = lc + dot(dc, J.ta);
     double n3 = lc + dot(dc, J.tb), e3 = la + dot(da, J.tb);
     const double h = 0.5 * EPS_PSI_THETA2;
-    if (fabs(e1) < h * lb) { n1 = lb; e1 = la; }
-    if (fabs(e2) < h * lc) { n2 = lc; e2 = lb; }
-    if (fabs(e3) < h * la) { n3 = la; e3 = lc; }
+    const bool f1 = fabs(e1) < h * lb, f2 = fabs(e2) < h * lc, f3 = fabs(e3) < h * la;
+    if (__builtin_expect(f1 | f2 | f3, 0)) {   // a point on an edge line of K_j: rare, skipped by whole wavefronts
+        if (f1) { n1 = lb; e1 = la; }
+        if (f2) { n2 = lc; e2 = lb; }
+        if (f3) { n3 = la; e3 = lc; }
+    }
     const double e12 = e1 * e2;
     const double inv = 1.0 / (e12 * e3);
-    const double t1 = i2m::log_pos(n1 * (e2 * e3) * inv);
-    const double t2 = i2m::log_pos(n2 * (e1 * e3) * inv);
-    const double t3 = i2m::log_pos(n3 * e12 * inv);
+    const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);
+    const double t2 = i2m::log_tab(n2 * (e1 * e3) * inv, lt);
+    const double t3 = i2m::log_tab(n3 * e12 * inv, lt);
     q4 r;
     r.x = t1 * J.tc.x + t2 * J.ta.x + t3 * J.tb.x;
     r.y = t1 * J.tc.y + t2 * J.ta.y + t3 * J.tb.y;

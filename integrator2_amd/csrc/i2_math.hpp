@@ -1,18 +1,24 @@
-// i2_math.hpp -- FP64 logarithm and two-argument arctangent for the
-// quadrature hot loop.
+// i2_math.hpp -- FP64 square root, logarithm and two-argument arctangent
+// for the quadrature hot loop.
 //
-// The library versions cost about 92 (log) and 99 (atan2) VALU instructions
-// on gfx950; the inner integral needs three logarithms and one atan2 per
-// quadrature point, so they dominate the kernel.  These versions use one
-// division each and a short polynomial, are branch-free on the normal path,
-// and stay within about 2 ulp (checked on the host against glibc and
-// high-precision references).
+// The library versions cost about 16 (sqrt), 92 (log) and 99 (atan2) VALU
+// instructions on gfx950; the inner integral needs three of each of the
+// first two and one atan2 per quadrature point, so they dominate the kernel.
+// These versions drop the range handling the hot loop never needs, are
+// branch-free on the normal path, and stay within about 2 ulp (checked on the
+// host against 120-bit references).
 //
 // Methods (textbook):
-//   log:   x = 2^k m with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s) with
-//          s = (m - 1)/(m + 1), |s| <= 0.1716; atanh(s) = s (1 + s^2/3 + s^4/5 + ...),
-//          truncated after s^18/19 (truncation below 2^-54 relative);
-//          log x = k ln2_hi + (2 atanh s + k ln2_lo), ln2 split so that k ln2_hi is exact.
+//   sqrt:  s > 0 normal: y ~ 1/sqrt(s) from the hardware estimate, one
+//          Goldschmidt step g = s y, h = y/2, r = 1/2 - g h, then
+//          g + (s - g^2) h (one correction; within 1 ulp, not always
+//          correctly rounded).
+//   log:   x = 2^k m with m in [sqrt(1/2), sqrt(2)); j = rint(64 (m - 1)) picks
+//          c_j = 1 + j/64 from a 47-entry table of invc_j = 1/c_j (rounded) and
+//          T_j = -log(invc_j) (hi + lo); r = m invc_j - 1 by one fma, |r| < 0.0112;
+//          log1p(r) by its Taylor series to r^8 (truncation below 2^-56
+//          relative); log x = (k ln2_hi + T_hi) + (k ln2_lo + T_lo + log1p r).
+//          c_0 = 1 exactly, so there is no cancellation near x = 1.  No division.
 //   atan2: reduce to t = min(|x|,|y|)/max(|x|,|y|) in [0, 1]; with a centre
 //          c in {0, c8, 1} (c8 = the double nearest tan(pi/8)),
 //          atan t = atan c + atan u, u = (n - c d)/(d + c n) for t = n/d, so
@@ -46,28 +52,52 @@ constexpr double T316 = 0.66817863791929888;           // tan(3 pi/16)
 static I2_HD __noinline__ double log_slow(double x) { return ::log(x); }
 static I2_HD __noinline__ double atan2_slow(double y, double x) { return ::atan2(y, x); }
 
-I2_HD __forceinline__ double log_pos(double x) {
+// estimate of 1/sqrt(s): the hardware instruction on the device, the library on the host
+__device__ __forceinline__ double rsq_estimate(double s) { return __builtin_amdgcn_rsq(s); }
+__host__ inline double rsq_estimate(double s) { return 1.0 / ::sqrt(s); }
+
+// sqrt of a positive normal number (squared distances of the hot loop)
+I2_HD __forceinline__ double sqrt_pos(double s) {
+    const double y = rsq_estimate(s);
+    double g = s * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    const double d = fma(-g, g, s);
+    return fma(d, h, g);
+}
+
+// log reduction table (gen_log_table.py): entry j + LOGTAB_J0 for c_j = 1 + j/64
+struct LogEntry { double invc, thi, tlo; };
+constexpr int LOGTAB_J0 = 19;
+constexpr int LOGTAB_N = 47;
+static const LogEntry h_logtab[LOGTAB_N] = {
+#include "i2_logtab.inc"
+};
+static __constant__ LogEntry c_logtab[LOGTAB_N] = {
+#include "i2_logtab.inc"
+};
+
+// log of a positive normal number with the reduction table `tab` (LDS on the device)
+I2_HD __forceinline__ double log_tab(double x, const LogEntry* __restrict__ tab) {
     if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log_slow(x);
     int k;
     double m = frexp(x, &k);             // x = m 2^k, m in [0.5, 1)
     if (m < SQRT_HALF) { m += m; --k; }  // m in [sqrt(1/2), sqrt(2))
-    const double f = m - 1.0;            // exact
-    const double s = f / (m + 1.0);
-    const double z = s * s;
-    // 1/3 + z/5 + ... + z^8/19
-    double p = 1.0 / 19.0;
-    p = fma(p, z, 1.0 / 17.0);
-    p = fma(p, z, 1.0 / 15.0);
-    p = fma(p, z, 1.0 / 13.0);
-    p = fma(p, z, 1.0 / 11.0);
-    p = fma(p, z, 1.0 / 9.0);
-    p = fma(p, z, 1.0 / 7.0);
-    p = fma(p, z, 1.0 / 5.0);
-    p = fma(p, z, 1.0 / 3.0);
-    const double s2 = s + s;
-    const double r = fma(s2 * z, p, s2);          // 2 atanh(s)
+    const int j = (int)rint(fma(m, 64.0, -64.0));
+    const LogEntry e = tab[j + LOGTAB_J0];
+    const double r = fma(m, e.invc, -1.0);
+    // log1p(r) = r - r^2/2 + ... - r^8/8
+    double q = -1.0 / 8.0;
+    q = fma(q, r, 1.0 / 7.0);
+    q = fma(q, r, -1.0 / 6.0);
+    q = fma(q, r, 1.0 / 5.0);
+    q = fma(q, r, -1.0 / 4.0);
+    q = fma(q, r, 1.0 / 3.0);
+    q = fma(q, r, -0.5);
+    const double p = fma(r * r, q, r);
     const double dk = (double)k;
-    return fma(dk, LN2_HI, fma(dk, LN2_LO, r));
+    return fma(dk, LN2_HI, e.thi) + (fma(dk, LN2_LO, e.tlo) + p);
 }
 
 I2_HD __forceinline__ double atan2_fast(double y, double x) {

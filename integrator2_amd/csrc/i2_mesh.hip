@@ -141,7 +141,8 @@ const char* i2_last_error(void) { return g_last_error.c_str(); }
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out) {
     if (!x || !out || n < 0 || (fn == 1 && !y)) { set_error("null argument"); return I2_ERR_INVALID; }
     for (int64_t q = 0; q < n; ++q) {
-        if (fn == 0) out[q] = i2m::log_pos(x[q]);
+        if (fn == 0) out[q] = i2m::log_tab(x[q], i2m::h_logtab);
+        else if (fn == 2) out[q] = i2m::sqrt_pos(x[q]);
         else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
         else { set_error("unknown function %d", fn); return I2_ERR_INVALID; }
     }
@@ -155,7 +156,7 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
     const JGeom J = {A, B, C, unit(C - B), unit(A - C), unit(B - A)};
     for (int64_t q = 0; q < n; ++q) {
         const d3 p = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
-        const q4 r = fast ? theta_psi_fast(p, J) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
+        const q4 r = fast ? theta_psi_fast(p, J, i2m::h_logtab) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
         out4[4 * q] = r.x; out4[4 * q + 1] = r.y; out4[4 * q + 2] = r.z; out4[4 * q + 3] = r.w;
     }
     return I2_OK;

@@ -69,6 +69,16 @@ def test_host_log_within_2_ulp(i2):
     assert abs(special[3] - np.log(5e-320)) < 1e-12
 
 
+def test_host_sqrt_within_1_ulp(i2):
+    import mpmath
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(2)
+    x = np.exp(rng.uniform(-600, 600, 4000))
+    got = _host_math(i2, 2, x)
+    e = _ulp_err(got, [mpmath.sqrt(mpmath.mpf(v)) for v in x])
+    assert e.max() <= 1.0, e.max()
+
+
 def test_host_atan2_within_2_ulp(i2):
     import mpmath
     mpmath.mp.prec = 120
@@ -226,7 +236,7 @@ def test_point_adjoints_match_central_differences(i2, kind):
         f, pb, parb = _point_adjoint(i2, kind, p, params, gbar)
         if kind == 0:
             ref = _host_theta_psi(i2, 1, p, params[:9])[0]
-            assert np.abs(f - ref).max() <= 1e-14 * np.abs(ref).max()   # tangents normalised by numpy here
+            assert np.abs(f - ref).max() <= 5e-14 * np.abs(ref).max()   # tangents normalised by numpy here
         h = 1e-6
         def phi(pp, qq):
             return _point_adjoint(i2, kind, pp, qq, gbar)[0] @ gbar

@@ -72,6 +72,7 @@ def test_config3_batch_row_sums(gpu, oracle_mod):
     for k, (lev, xyz, tri) in enumerate(specs):
         if lev != 2:
             continue
+        print(f"mesh {k}: oracle check", flush=True)
         orc = oracle_mod.Oracle(xyz, tri)
         single = gpu.Batch(0)
         single.addMesh(meshes[k])

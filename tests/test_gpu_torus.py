@@ -1,0 +1,107 @@
+"""Forward and adjoint in one captured graph (batched path, row-sum outputs):
+the torus with major radius 1, minor radius 0.25 on a 256 x 128 grid
+(65,536 triangles, 4,294,901,760 ordered pairs), normal jitter 0.02 of the
+mean edge (seed 0), refinement up to 2 levels at tolerance 1e-8.
+
+Small meshes first: the batch graph's loss and gradient against the
+evaluator's separately run adjoint (checked against the oracle in
+test_gpu_adjoint), a small torus against the oracle's dual numbers and
+finite differences, and graph replays.  Then the full torus: its gradient at
+one vertex against the oracle's dual numbers over the ~786,000 ordered pairs
+that touch it (the pairs' levels from the oracle's own decisions with the
+GPU's K)."""
+import numpy as np
+import pytest
+
+from tests.helpers import reported_levels
+from tests.test_gpu_adjoint import check_adjoint
+from tests.test_gpu_parity import run_gpu
+
+pytestmark = pytest.mark.gpu
+SEED = 1
+
+
+def prepared(gpu, xyz, tri):
+    m = gpu.Mesh3D(0)
+    m.setData(xyz, tri)
+    m.prepareMesh()
+    return m
+
+
+def test_batch_graph_adjoint_matches_evaluator(gpu):
+    specs = [gpu.torus(32, 16, 1.0, 0.25, 0.02, 0), gpu.icosphere(2, (2.0, 1.0, 0.6), 0.05, 7)]
+    meshes = [prepared(gpu, *s) for s in specs]
+    integ = gpu.NumericalIntegrator3D(meshes[0])
+    integ.setErrorControl(2, 1e-8, True)
+    b = gpu.Batch(0)
+    for m in meshes:
+        b.addMesh(m)
+    b.configure(integ, gpu.BATCH_ROW_SUMS)
+    b.run()
+    rows_plain = [b.rowSums(k) for k in range(2)]
+    b.enableAdjoint(True, SEED)
+    runs = []
+    for rep in range(2):
+        b.run()
+        runs.append([b.lossAndGrad(k) for k in range(2)])
+        for k in range(2):
+            r = b.rowSums(k)
+            assert np.abs(r - rows_plain[k]).max() <= 1e-13 * np.abs(rows_plain[k]).max()
+    for k, (xyz, tri) in enumerate(specs):
+        objs, _, _ = run_gpu(gpu, xyz, tri, maxLevel=2, tol=1e-8, quirk=True, path=1, check=False)
+        L_ref, G_ref = objs[2].lossAndGrad(SEED)
+        for rep in range(2):
+            L, G = runs[rep][k]
+            print(f"mesh {k} replay {rep}: L {L:.12e} vs evaluator {L_ref:.12e}", flush=True)
+            assert abs(L - L_ref) <= 1e-10 * abs(L_ref)
+            assert np.abs(G - G_ref).max() <= 1e-9 * np.abs(G_ref).max()
+
+
+def test_small_torus_adjoint_vs_oracle(gpu, oracle_mod):
+    xyz, tri = gpu.torus(32, 16, 1.0, 0.25, 0.02, 0)
+    check_adjoint(gpu, oracle_mod, xyz, tri, "torus 32x16", ncoords=24, maxLevel=2, tol=1e-8, quirk=True, path=1)
+
+
+def test_full_torus_forward_and_adjoint_in_one_graph(gpu, oracle_mod):
+    xyz, tri = gpu.torus(256, 128, 1.0, 0.25, 0.02, 0)
+    mesh = prepared(gpu, xyz, tri)
+    integ = gpu.NumericalIntegrator3D(mesh)
+    integ.setErrorControl(2, 1e-8, True)
+    b = gpu.Batch(0)
+    b.addMesh(mesh)
+    b.configure(integ, gpu.BATCH_ROW_SUMS)
+    b.enableAdjoint(True, SEED)
+    print("torus: running forward + adjoint graph", flush=True)
+    ms = b.run()
+    n = b.taskCount()
+    Ks = [b.refinementIterations(0, t) for t in range(3)]
+    L, G = b.lossAndGrad(0)
+    print(f"torus: {n} ordered pairs, forward + adjoint {ms:.0f} ms, K {Ks}, L {L:.6e}, |G| {np.linalg.norm(G):.3e}",
+          flush=True)
+    assert n == 4294901760
+    assert all(1 <= k <= 2 for k in Ks)
+    assert np.isfinite(L) and np.isfinite(G).all()
+    rows = b.rowSums(0)
+    assert np.isfinite(rows).all()
+    # gradient at a vertex against the oracle's dual numbers over the pairs touching it
+    orc = oracle_mod.Oracle(xyz, tri)
+    vtx = int(np.random.default_rng(SEED).integers(len(xyz)))
+    cells = np.nonzero((tri == vtx).any(axis=1))[0]
+    others = np.arange(len(tri), dtype=np.int32)
+    ij = np.unique(np.concatenate([np.stack(np.broadcast_arrays(np.int32(c), others), axis=1) for c in cells] +
+                                  [np.stack(np.broadcast_arrays(others, np.int32(c)), axis=1) for c in cells]), axis=0)
+    ij = ij[ij[:, 0] != ij[:, 1]]
+    shared = (tri[ij[:, 0]][:, :, None] == tri[ij[:, 1]][:, None, :]).any(axis=2).sum(axis=1)
+    print(f"torus: oracle dual numbers over {len(ij)} pairs touching vertex {vtx}", flush=True)
+    for axis in range(3):
+        d = 0.0
+        for t, count in ((0, 1), (1, 2), (2, 0)):
+            sel = np.ascontiguousarray(ij[shared == count])
+            _, _, conv, _ = orc.run_tasks(t, sel, gpu.QF3D13_L12, gpu.QF3D13_W, 7, mode=1, level=2, tol=1e-8,
+                                          K_override=Ks[t])
+            rl = reported_levels(conv, Ks[t], True)
+            d += orc.loss_tasks(t, sel, rl, gpu.QF3D13_L12, gpu.QF3D13_W, 7, SEED, vtx=vtx, axis=axis, dual=True)
+            print(f"  axis {axis} type {t}: {len(sel)} pairs done", flush=True)
+        print(f"torus vertex {vtx} axis {axis} ({len(ij)} pairs): adjoint {G[vtx, axis]:.12e}, oracle dual {d:.12e}",
+              flush=True)
+        assert abs(G[vtx, axis] - d) <= 1e-6 * max(abs(d), 1e-3 * np.abs(G).max())
