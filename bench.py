@@ -53,6 +53,10 @@ BASIC_OPS_PER_POINT = 188
 TRANS_PER_POINT = {"sqrt": 6, "log": 3, "atan2": 1}
 TRANS_WEIGHTS = {"sqrt": 16, "log": 92, "atan2": 99}
 POINTS = 13
+# HBM bytes per sweep dispatch on this workload, measured with rocprofv3
+# FETCH_SIZE and WRITE_SIZE and scaled by a 1 GiB copy's readings
+# (201 MB read + 2.094 GB written; the algorithmic figure is 1.70 GB)
+SWEEP_HBM_BYTES = 2.295e9
 
 
 def flops_per_point():
@@ -248,7 +252,9 @@ def main():
         "plain_ms_per_step": 1e3 * t_plain,
         "batched_over_plain": t_plain / t_fast,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": SWEEP_HBM_BYTES,
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), copy-calibrated; "
+                                       "profiles/valu_counters_sweep_icosphere4.md",
                      "kernel": "k_sweep_disjoint (levels 0 and 1 of all disjoint pairs)", "kernel_ms": sweep_ms,
                      "flops_per_point": flops_per_point(), "points": points,
                      "algorithmic_bytes": traffic_bytes},
