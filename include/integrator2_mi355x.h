@@ -57,6 +57,15 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
  * x; the host uses the library's 1/sqrt as the estimate the device takes
  * from the hardware).  For checking. */
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out);
+/* The same functions evaluated on the device (fn 0-2 as above; 3, 4, 5: the
+ * HIP library's log, atan2 and sqrt, for comparison).  For checking. */
+int i2_device_math(int device, int fn, const double* x, const double* y, int64_t n, double* out);
+/* Host-side evaluation of one ordered task's integrals (Psi, Theta) at
+ * refinement level `level` with the kernels' device code compiled for the
+ * host: I9, J9 the two panels' vertices, vid6 their vertex ids (which decide
+ * the neighbour type).  For checking. */
+int i2_host_task_value(const double* I9, const double* J9, const int32_t* vid6, int level, const double* L12,
+                       const double* w, int nPts, double* out4);
 
 /* ======================================================================
  * Mesh3D (src/Mesh3d.cuh:55-126, src/Mesh3d.cu)

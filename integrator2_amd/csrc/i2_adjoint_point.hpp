@@ -23,16 +23,9 @@ I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::Log
     // ---- forward, as theta_psi_fast
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
     const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
-    double n1 = la + dot(da, J.tc), e1 = lb + dot(db, J.tc);
-    double n2 = lb + dot(db, J.ta), e2 = lc + dot(dc, J.ta);
-    double n3 = lc + dot(dc, J.tb), e3 = la + dot(da, J.tb);
-    const double h = 0.5 * EPS_PSI_THETA2;
-    const bool f1 = fabs(e1) < h * lb, f2 = fabs(e2) < h * lc, f3 = fabs(e3) < h * la;
-    if (__builtin_expect(f1 | f2 | f3, 0)) {
-        if (f1) { n1 = lb; e1 = la; }
-        if (f2) { n2 = lc; e2 = lb; }
-        if (f3) { n3 = la; e3 = lc; }
-    }
+    double n1, e1, n2, e2, n3, e3;
+    bool f1, f2, f3;
+    theta_psi_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
     const double e12 = e1 * e2;
     const double inv = 1.0 / (e12 * e3);
     const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);

@@ -134,20 +134,50 @@ I2_HD __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, 
 // i2_math.hpp, within 2 ulp; `lt` is the log reduction table, in LDS on the
 // device).  The degenerate-branch tests |1 + (d/l).tau| < eps become
 // |l + d.tau| < eps l.
+//
+// Near an edge line of K_j (in particular next to a vertex that K_i shares
+// with K_j) l + d.tau cancels: with the point at distance delta from the line
+// it is ~delta^2/(2l), and the sum loses l^2/delta^2 ulps, as the reference's
+// 1 + cos does.  Terms below 1e-4 l are recomputed as |d x tau|^2/(l - d.tau),
+// the same number without cancellation, so these points come out more
+// accurate than the reference formula.  The branch is rare for disjoint pairs
+// and skipped by whole wavefronts.
 struct JGeom { d3 a, b, c, ta, tb, tc; };
-I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogEntry* __restrict__ lt) {
-    const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
-    const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
-    double n1 = la + dot(da, J.tc), e1 = lb + dot(db, J.tc);
-    double n2 = lb + dot(db, J.ta), e2 = lc + dot(dc, J.ta);
-    double n3 = lc + dot(dc, J.tb), e3 = la + dot(da, J.tb);
-    const double h = 0.5 * EPS_PSI_THETA2;
-    const bool f1 = fabs(e1) < h * lb, f2 = fabs(e2) < h * lc, f3 = fabs(e3) < h * la;
-    if (__builtin_expect(f1 | f2 | f3, 0)) {   // a point on an edge line of K_j: rare, skipped by whole wavefronts
+I2_HD __forceinline__ double l_plus_dtau(double l, d3 d, d3 t) {
+    const d3 c = cross(d, t);
+    return dot(c, c) / (l - dot(d, t));
+}
+// the six l + d.tau terms of the inner integral and the reference's degenerate branch
+I2_HD __forceinline__ void theta_psi_terms(const JGeom& J, d3 da, d3 db, d3 dc, double la, double lb, double lc,
+                                           double& n1, double& e1, double& n2, double& e2, double& n3, double& e3,
+                                           bool& f1, bool& f2, bool& f3) {
+    n1 = la + dot(da, J.tc); e1 = lb + dot(db, J.tc);
+    n2 = lb + dot(db, J.ta); e2 = lc + dot(dc, J.ta);
+    n3 = lc + dot(dc, J.tb); e3 = la + dot(da, J.tb);
+    const double s = 1e-4;
+    const bool c1 = n1 < s * la, c2 = e1 < s * lb, c3 = n2 < s * lb, c4 = e2 < s * lc, c5 = n3 < s * lc, c6 = e3 < s * la;
+    f1 = f2 = f3 = false;
+    if (__builtin_expect(c1 | c2 | c3 | c4 | c5 | c6, 0)) {
+        if (c1) n1 = l_plus_dtau(la, da, J.tc);
+        if (c2) e1 = l_plus_dtau(lb, db, J.tc);
+        if (c3) n2 = l_plus_dtau(lb, db, J.ta);
+        if (c4) e2 = l_plus_dtau(lc, dc, J.ta);
+        if (c5) n3 = l_plus_dtau(lc, dc, J.tb);
+        if (c6) e3 = l_plus_dtau(la, da, J.tb);
+        // a point on an edge line of K_j keeps the ratio of distances (evaluatorJ3DK.cu:294-307)
+        const double h = 0.5 * EPS_PSI_THETA2;
+        f1 = fabs(e1) < h * lb; f2 = fabs(e2) < h * lc; f3 = fabs(e3) < h * la;
         if (f1) { n1 = lb; e1 = la; }
         if (f2) { n2 = lc; e2 = lb; }
         if (f3) { n3 = la; e3 = lc; }
     }
+}
+I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogEntry* __restrict__ lt) {
+    const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
+    const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
+    double n1, e1, n2, e2, n3, e3;
+    bool f1, f2, f3;
+    theta_psi_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
     const double e12 = e1 * e2;
     const double inv = 1.0 / (e12 * e3);
     const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);

@@ -75,15 +75,20 @@ template <> struct PairCtx<I2_SIMPLE_NEIGHBORS> {
 };
 
 // Quadrature over one sub-panel: area * sum_k w_k (thetaPsi - singular part)(r_k)
-// (NumericalIntegrator3d.cu:560-585, evaluatorJ3DK.cu:98-106)
+// (NumericalIntegrator3d.cu:560-585, evaluatorJ3DK.cu:98-106).  thetaPsi is
+// the cancellation-free form shared with the batched path: the reference's
+// own FP64 form loses up to 1e-7 of a value to cancellation where a
+// quadrature point lies near the extension of an edge line of K_j, which
+// moves Runge decisions; both paths therefore take decisions on accurate values.
 template <int TYPE>
-__device__ __forceinline__ q4 quad_sub(const Rule& R, d3 a, d3 b, d3 c, double area, const Panel& J, const PairCtx<TYPE>& ctx) {
+__device__ __forceinline__ q4 quad_sub(const Rule& R, const i2m::LogEntry* lt, d3 a, d3 b, d3 c, double area,
+                                       const JGeom& J, const PairCtx<TYPE>& ctx) {
     q4 acc = q4zero();
     for (int k = 0; k < R.n; ++k) {
         d3 p = R.L1[k] * a;
         p = p + R.L2[k] * b;
         p = p + R.L3[k] * c;
-        q4 f = theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
+        q4 f = theta_psi_fast(p, J, lt);
         if (TYPE != I2_NOT_NEIGHBORS) f = f - ctx.sing(p);
         acc = acc + R.w[k] * f;
     }
@@ -96,6 +101,9 @@ template <int TYPE, bool IMPLICIT>
 __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __restrict__ list, TaskSource src,
                                                const Panel* __restrict__ panels, Rule R, int level, int lanes_log2,
                                                double4* __restrict__ out) {
+    __shared__ i2m::LogEntry lt[i2m::LOGTAB_N];
+    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) lt[k] = i2m::c_logtab[k];
+    __syncthreads();
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int G = 1 << lanes_log2;
     const int64_t slot = gid >> lanes_log2;
@@ -109,6 +117,7 @@ __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __
         task_ij<IMPLICIT>(src, z, &i, &j);
         const Panel I = panels[i];
         const Panel J = panels[j];
+        const JGeom JG{J.a, J.b, J.c, J.ta, J.tb, J.tc};
         PairCtx<TYPE> ctx;
         ctx.init(I, J);
         double area = I.area;
@@ -117,7 +126,7 @@ __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __
         for (uint32_t c = sub; c < nsub; c += G) {
             d3 a = I.a, b = I.b, cc = I.c;
             sub_panel(a, b, cc, level, c);
-            acc = acc + quad_sub<TYPE>(R, a, b, cc, area, J, ctx);
+            acc = acc + quad_sub<TYPE>(R, lt, a, b, cc, area, JG, ctx);
         }
     }
     for (int m = 1; m < G; m <<= 1) {

@@ -162,6 +162,107 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
     return I2_OK;
 }
 
+// The kernels' math functions on the device, for checking (fn as in i2_host_math).
+__global__ void k_math(int fn, int64_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out) {
+    __shared__ i2m::LogEntry lt[i2m::LOGTAB_N];
+    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) lt[k] = i2m::c_logtab[k];
+    __syncthreads();
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    if (fn == 0) out[q] = i2m::log_tab(x[q], lt);
+    else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
+    else if (fn == 2) out[q] = i2m::sqrt_pos(x[q]);
+    else if (fn == 3) out[q] = log(x[q]);
+    else if (fn == 4) out[q] = atan2(y[q], x[q]);
+    else out[q] = sqrt(x[q]);
+}
+
+// Host evaluation of one ordered task's integrals with the kernels' device
+// code compiled for the host: level-`level` quadrature of (thetaPsi -
+// singular part) over K_i's sub-panels (the batched path's point and
+// sub-panel arithmetic, the division-free inner integral) plus the closed-form
+// singular part.  tri9 are the two panels' vertices, vid6 their vertex ids
+// (they decide the neighbour type and the rotations).  For checking.
+int i2_device_math(int device, int fn, const double* x, const double* y, int64_t n, double* out) {
+    if (!x || !out || n <= 0 || fn < 0 || fn > 5 || ((fn == 1 || fn == 4) && !y)) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    DeviceGuard g(device);
+    if (!g.ok) return I2_ERR_HIP;
+    DBuf<double> dx, dy, dout;
+    I2_TRY(dx.ensure(n)); I2_TRY(dy.ensure(n)); I2_TRY(dout.ensure(n));
+    I2_HIP(hipMemcpy(dx.p, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    I2_HIP(hipMemcpy(dy.p, y ? y : x, sizeof(double) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, fn, n, dx.p, dy.p, dout.p);
+    I2_TRY(check_launch("k_math"));
+    I2_HIP(hipMemcpy(out, dout.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+int i2_host_task_value(const double* I9, const double* J9, const int32_t* vid6, int level, const double* L12,
+                       const double* w, int nPts, double* out4) {
+    if (!I9 || !J9 || !vid6 || !L12 || !w || !out4 || nPts < 1 || nPts > i2d::MAX_POINTS || level < 0 || level > 12) {
+        set_error("invalid argument");
+        return I2_ERR_INVALID;
+    }
+    using namespace i2d;
+    auto panel = [](const double* X, const int32_t* v) {
+        Panel P{};
+        P.a = {X[0], X[1], X[2]}; P.b = {X[3], X[4], X[5]}; P.c = {X[6], X[7], X[8]};
+        P.ta = unit(P.c - P.b); P.tb = unit(P.a - P.c); P.tc = unit(P.b - P.a);
+        const d3 cr = cross(P.b - P.a, P.c - P.a);
+        P.n = unit(cr);
+        P.area = vlen(cr) * 0.5;
+        P.v0 = v[0]; P.v1 = v[1]; P.v2 = v[2];
+        return P;
+    };
+    const Panel I = panel(I9, vid6), J = panel(J9, vid6 + 3);
+    const int shared = shared_count(I, J);
+    const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+    int s1 = 0, s2 = 0;
+    EdgeCtx ec{};
+    VertexCtx vc{};
+    if (shared == 2) { shifts_edge(I, J, &s1, &s2); ec = edge_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3)); }
+    if (shared == 1) {
+        shifts_vertex(I, J, &s1, &s2);
+        vc = vertex_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    }
+    double area = I.area;
+    for (int l = 0; l < level; ++l) area *= 0.25;
+    q4 total = q4zero();
+    const uint32_t nsub = 1u << (2 * level);
+    for (uint32_t c = 0; c < nsub; ++c) {
+        d3 a = I.a, b = I.b, cc = I.c;
+        if (level == 1) {   // the sweep kernel's level-1 children
+            const d3 mBC = 0.5 * (I.b + I.c), mCA = 0.5 * (I.c + I.a), mAB = 0.5 * (I.a + I.b);
+            if (c == 0) { a = mAB; b = I.b; cc = mBC; }
+            else if (c == 1) { a = mBC; b = I.c; cc = mCA; }
+            else if (c == 2) { a = mCA; b = I.a; cc = mAB; }
+            else { a = mBC; b = mCA; cc = mAB; }
+        } else {
+            sub_panel(a, b, cc, level, c);
+        }
+        q4 acc = q4zero();
+        for (int k = 0; k < nPts; ++k) {
+            const double l1 = L12[2 * k], l2 = L12[2 * k + 1], l3 = 1.0 - l1 - l2;
+            d3 p = l1 * a;
+            p = p + l2 * b;
+            p = p + l3 * cc;
+            q4 f = theta_psi_fast(p, JG, i2m::h_logtab);
+            if (shared == 2) f = f - sing_edge(p, ec);
+            if (shared == 1) f = f - sing_vertex(p, vc);
+            acc = acc + w[k] * f;
+        }
+        total = total + area * acc;
+    }
+    if (shared == 2)
+        total = total + integrate_sing_edge(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                           vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    if (shared == 1)
+        total = total + integrate_sing_vertex(vtx(I, s1), vtx(I, (s1 + 1) % 3), vtx(I, (s1 + 2) % 3),
+                                             vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
+    out4[0] = total.x; out4[1] = total.y; out4[2] = total.z; out4[3] = total.w;
+    return I2_OK;
+}
+
 int i2_device_count(int* n) {
     if (!n) { set_error("null pointer"); return I2_ERR_INVALID; }
     int c = 0;

@@ -81,8 +81,11 @@ class Oracle:
         return out
 
     def run_tasks(self, t, ij, L12, w, order, mode=1, level=5, tol=1e-5, quirk=True, K_override=-1,
-                  nthreads=0, want_levels=False):
-        """Integrals (n,4), J (n,3), conv (n,), K [, levels (n, level+1, 4)] for ordered tasks ij (n,2)."""
+                  nthreads=0, want_levels=False, long_double=False):
+        """Integrals (n,4), J (n,3), conv (n,), K [, levels (n, level+1, 4)] for ordered tasks ij (n,2).
+
+        long_double=True evaluates the level values and the singular part in
+        long double (each rounded once to double) before the Runge decisions."""
         ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
         n = len(ij)
         L12 = np.ascontiguousarray(L12, np.float64)
@@ -92,10 +95,10 @@ class Oracle:
         conv = np.zeros(n, np.int8)
         lev = np.zeros((n, level + 1, 4)) if want_levels else None
         K = ctypes.c_int()
-        lib().o_run_tasks(self._h, ctypes.c_int(t), _ptr(ij), _I64(n), _ptr(L12), _ptr(w), ctypes.c_int(len(w)),
-                          ctypes.c_int(order), ctypes.c_int(mode), ctypes.c_int(level), _D(tol), ctypes.c_int(int(quirk)),
-                          ctypes.c_int(K_override), ctypes.c_int(nthreads), _ptr(I4), _ptr(J3), _ptr(conv), _ptr(lev),
-                          ctypes.byref(K))
+        lib().o_run_tasks_prec(self._h, ctypes.c_int(t), _ptr(ij), _I64(n), _ptr(L12), _ptr(w), ctypes.c_int(len(w)),
+                               ctypes.c_int(order), ctypes.c_int(mode), ctypes.c_int(level), _D(tol),
+                               ctypes.c_int(int(quirk)), ctypes.c_int(K_override), ctypes.c_int(nthreads),
+                               ctypes.c_int(int(long_double)), _ptr(I4), _ptr(J3), _ptr(conv), _ptr(lev), ctypes.byref(K))
         if want_levels:
             return I4, J3, conv, K.value, lev
         return I4, J3, conv, K.value
@@ -134,16 +137,18 @@ class Oracle:
         d, ld = self.singular_integrals(t, ij)
         return np.abs(d - ld).max(axis=1) / np.maximum(np.abs(np.asarray(totals)).max(axis=1), 1e-300)
 
-    def values_at_levels(self, t, ij, rlevel, L12, w, order):
-        """Integrals (n,4) and J (n,3) of ordered tasks at the given reported levels."""
+    def values_at_levels(self, t, ij, rlevel, L12, w, order, long_double=False):
+        """Integrals (n,4) and J (n,3) of ordered tasks at the given reported levels
+        (evaluated in long double when long_double)."""
         ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
         rlevel = np.ascontiguousarray(rlevel, np.int8)
         L12 = np.ascontiguousarray(L12, np.float64)
         w = np.ascontiguousarray(w, np.float64)
         I4 = np.zeros((len(ij), 4))
         J3 = np.zeros((len(ij), 3))
-        lib().o_tasks_at_levels(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12), _ptr(w),
-                                ctypes.c_int(len(w)), ctypes.c_int(order), _ptr(I4), _ptr(J3))
+        lib().o_tasks_at_levels_prec(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12),
+                                     _ptr(w), ctypes.c_int(len(w)), ctypes.c_int(order), ctypes.c_int(int(long_double)),
+                                     _ptr(I4), _ptr(J3))
         return I4, J3
 
     def level_value(self, t, i, j, L12, w, order, level):
@@ -193,3 +198,20 @@ def _task_noise(self, t, ij, rlevel, L12, w, order, results=False):
 
 
 Oracle.task_noise = _task_noise
+
+
+def _task_noise_mca(self, t, ij, rlevel, L12, w, order, results=False, nrep=3):
+    """Monte Carlo arithmetic noise (every operation randomly perturbed by up to
+    one ulp) of tasks at their reported levels, relative to the integrals or J."""
+    ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
+    rlevel = np.ascontiguousarray(rlevel, np.int8)
+    L12 = np.ascontiguousarray(L12, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    out = np.zeros(len(ij))
+    out_j = np.zeros(len(ij))
+    lib().o_task_noise_mca(self._h, ctypes.c_int(t), _ptr(ij), _ptr(rlevel), _I64(len(ij)), _ptr(L12), _ptr(w),
+                           ctypes.c_int(len(w)), ctypes.c_int(order), ctypes.c_int(nrep), _ptr(out), _ptr(out_j))
+    return out_j if results else out
+
+
+Oracle.task_noise_mca = _task_noise_mca

@@ -302,9 +302,13 @@ void o_level_value(void* h, int type, int i, int j, const double* L12, const dou
 // replaces K (used when only a sample of a type's tasks is evaluated).
 // Outputs per task: integrals (4), results J (3), conv = convergence iteration
 // c (int8), and optionally all level values I_0..I_c (levels4, stride 4*(maxL+1)).
-int o_run_tasks(void* h, int type, const int32_t* ij, int64_t n, const double* L12, const double* w, int nPts, int order,
-                int mode, int level, double tol, int quirk, int K_override, int nthreads,
-                double* out_integrals4, double* out_results3, int8_t* out_conv, double* out_levels4, int* K_out) {
+// prec != 0 evaluates every level value and the singular part in long double
+// and rounds each once to double, so that the Runge decisions are taken on
+// values free of the FP64 cancellation of the reference formula near the
+// extension of an edge line; the criterion itself stays in double.
+int o_run_tasks_prec(void* h, int type, const int32_t* ij, int64_t n, const double* L12, const double* w, int nPts,
+                     int order, int mode, int level, double tol, int quirk, int K_override, int nthreads, int prec,
+                     double* out_integrals4, double* out_results3, int8_t* out_conv, double* out_levels4, int* K_out) {
     const Mesh& m = *(Mesh*)h;
     Rule R = make_rule(L12, w, nPts, order);
     const double pow2p = (double)(1 << order);
@@ -319,17 +323,27 @@ int o_run_tasks(void* h, int type, const int32_t* ij, int64_t n, const double* L
         TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
         PairEval<double> E = pair_eval(m, type, i, j, g);
         SubPanel<double> P{g.I[0], g.I[1], g.I[2], g.areaI};
+        TaskGeom<long double> gl{};
+        PairEval<long double> El{};
+        if (prec) { gl = task_geom<long double>(m, i, j, nullptr, -1); El = pair_eval(m, type, i, j, gl); }
+        const SubPanel<long double> Pl{gl.I[0], gl.I[1], gl.I[2], gl.areaI};
+        // level value in double, or in long double rounded to double once
+        auto lv = [&](int L) -> Q4<double> {
+            if (!prec) return level_value(R, P, L, E);
+            const Q4<long double> v = level_value(R, Pl, L, El);
+            return {(double)v.x, (double)v.y, (double)v.z, (double)v.w};
+        };
         if (mode == 0) {
-            Q4<double> v = level_value(R, P, level, E);
+            Q4<double> v = lv(level);
             finalv[q] = v; prevv[q] = v; conv[q] = (int8_t)level;
             if (out_levels4) { double* o = out_levels4 + (size_t)q * 4 * (maxL + 1) + 4 * level; o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
             continue;
         }
-        Q4<double> prev = level_value(R, P, 0, E), cur = prev;
+        Q4<double> prev = lv(0), cur = prev;
         if (out_levels4) { double* o = out_levels4 + (size_t)q * 4 * (maxL + 1); o[0] = prev.x; o[1] = prev.y; o[2] = prev.z; o[3] = prev.w; }
         int c = 0;
         for (int it = 1; it <= maxL; ++it) {
-            cur = level_value(R, P, it, E);
+            cur = lv(it);
             if (out_levels4) { double* o = out_levels4 + (size_t)q * 4 * (maxL + 1) + 4 * it; o[0] = cur.x; o[1] = cur.y; o[2] = cur.z; o[3] = cur.w; }
             c = it;
             if (!runge_unconverged(cur, prev, pow2p, tol)) break;
@@ -349,13 +363,25 @@ int o_run_tasks(void* h, int type, const int32_t* ij, int64_t n, const double* L
         Q4<double> v = finalv[q];
         if (mode == 1 && quirk && conv[q] >= 1 && ((K - conv[q]) & 1)) v = prevv[q];
         TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
-        v = v + singular_integral(m, type, i, j, g);
+        if (prec) {
+            const Q4<long double> s = singular_integral(m, type, i, j, task_geom<long double>(m, i, j, nullptr, -1));
+            v = v + Q4<double>{(double)s.x, (double)s.y, (double)s.z, (double)s.w};
+        } else {
+            v = v + singular_integral(m, type, i, j, g);
+        }
         V3<double> J = (type == VERTEX_SHARING) ? finalize_vertex(v, g.nJ, g.areaI) : finalize_plain(v, g.nJ);
         if (out_integrals4) { double* o = out_integrals4 + 4 * q; o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
         if (out_results3) { double* o = out_results3 + 3 * q; o[0] = J.x; o[1] = J.y; o[2] = J.z; }
         if (out_conv) out_conv[q] = conv[q];
     }
     return 0;
+}
+
+int o_run_tasks(void* h, int type, const int32_t* ij, int64_t n, const double* L12, const double* w, int nPts, int order,
+                int mode, int level, double tol, int quirk, int K_override, int nthreads,
+                double* out_integrals4, double* out_results3, int8_t* out_conv, double* out_levels4, int* K_out) {
+    return o_run_tasks_prec(h, type, ij, n, L12, w, nPts, order, mode, level, tol, quirk, K_override, nthreads, 0,
+                            out_integrals4, out_results3, out_conv, out_levels4, K_out);
 }
 
 // evaluator3d.cu:45-57 -- symmetry error for a task list laid out as
@@ -453,11 +479,46 @@ double o_loss_tasks(void* h, int type, const int32_t* ij, const int8_t* rlevel, 
     return o_loss_tasks_prec(h, type, ij, rlevel, n, L12, w, nPts, order, seed, vtx, axis, hstep, dual, 0, nthreads);
 }
 
+// Monte Carlo arithmetic noise of tasks at their reported levels: the
+// largest deviation over `nrep` randomly rounded evaluations from the plain
+// FP64 value, relative to the task's integrals (out) and to its J (out_j).
+void o_task_noise_mca(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n, const double* L12,
+                      const double* w, int nPts, int order, int nrep, double* out, double* out_j) {
+    const Mesh& m = *(Mesh*)h;
+    Rule R = make_rule(L12, w, nPts, order);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < n; ++q) {
+        const int i = ij[2 * q], j = ij[2 * q + 1];
+        TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
+        const Q4<double> v = task_level_value(m, R, type, i, j, rlevel[q], g) + singular_integral(m, type, i, j, g);
+        int p = 0;
+        const V3<double> J = (type == VERTEX_SHARING) ? finalize_vertex(v, g.nJ, g.areaI, &p) : finalize_plain(v, g.nJ);
+        const double s = std::max(std::max(std::fabs(v.x), std::fabs(v.y)), std::max(std::fabs(v.z), std::fabs(v.w)));
+        const double sj = std::max(std::fabs(J.x), std::max(std::fabs(J.y), std::fabs(J.z)));
+        double d = 0.0, dj = 0.0;
+        for (int r = 0; r < nrep; ++r) {
+            mca_state() = 0x1234567ull * (uint64_t)(q + 1) + 0x9E37ull * (uint64_t)(r + 1);
+            TaskGeom<MCA> gm = task_geom<MCA>(m, i, j, nullptr, -1);
+            const Q4<MCA> vm = task_level_value(m, R, type, i, j, rlevel[q], gm) + singular_integral(m, type, i, j, gm);
+            const MCA refT = MCA(TWO_PI) * gm.areaI;
+            const V3<MCA> Jm = (type == VERTEX_SHARING)
+                                   ? MCA(RECIPROCAL_FOUR_PI) * ((vm.w + MCA(2.0 * p) * refT) * gm.nJ + cross(vecof(vm), gm.nJ))
+                                   : finalize_plain(vm, gm.nJ);
+            d = std::max(d, std::max(std::max(std::fabs(vm.x.v - v.x), std::fabs(vm.y.v - v.y)),
+                                     std::max(std::fabs(vm.z.v - v.z), std::fabs(vm.w.v - v.w))));
+            dj = std::max(dj, std::max(std::fabs(Jm.x.v - J.x), std::max(std::fabs(Jm.y.v - J.y), std::fabs(Jm.z.v - J.z))));
+        }
+        out[q] = d / std::max(s, 1e-300);
+        if (out_j) out_j[q] = dj / std::max(sj, 1e-300);
+    }
+}
+
 // Integrals (numerical part at level rlevel[q] plus the closed singular part)
 // and J of ordered tasks at given levels, for comparing values when the
 // refinement decisions of the compared runs are imposed.
-void o_tasks_at_levels(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n, const double* L12,
-                       const double* w, int nPts, int order, double* out4, double* out3) {
+// prec = 1: the same in long double (J with the wrap count of the double value).
+void o_tasks_at_levels_prec(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n, const double* L12,
+                            const double* w, int nPts, int order, int prec, double* out4, double* out3) {
     const Mesh& m = *(Mesh*)h;
     Rule R = make_rule(L12, w, nPts, order);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -465,10 +526,25 @@ void o_tasks_at_levels(void* h, int type, const int32_t* ij, const int8_t* rleve
         const int i = ij[2 * q], j = ij[2 * q + 1];
         TaskGeom<double> g = task_geom<double>(m, i, j, nullptr, -1);
         const Q4<double> v = task_level_value(m, R, type, i, j, rlevel[q], g) + singular_integral(m, type, i, j, g);
-        const V3<double> J = (type == VERTEX_SHARING) ? finalize_vertex(v, g.nJ, g.areaI) : finalize_plain(v, g.nJ);
-        out4[4 * q] = v.x; out4[4 * q + 1] = v.y; out4[4 * q + 2] = v.z; out4[4 * q + 3] = v.w;
-        out3[3 * q] = J.x; out3[3 * q + 1] = J.y; out3[3 * q + 2] = J.z;
+        int p = 0;
+        const V3<double> J = (type == VERTEX_SHARING) ? finalize_vertex(v, g.nJ, g.areaI, &p) : finalize_plain(v, g.nJ);
+        if (!prec) {
+            out4[4 * q] = v.x; out4[4 * q + 1] = v.y; out4[4 * q + 2] = v.z; out4[4 * q + 3] = v.w;
+            out3[3 * q] = J.x; out3[3 * q + 1] = J.y; out3[3 * q + 2] = J.z;
+            continue;
+        }
+        TaskGeom<long double> gl = task_geom<long double>(m, i, j, nullptr, -1);
+        const Q4<long double> vl = task_level_value(m, R, type, i, j, rlevel[q], gl) + singular_integral(m, type, i, j, gl);
+        const long double refT = (long double)TWO_PI * gl.areaI;
+        const V3<long double> Jl = (long double)RECIPROCAL_FOUR_PI *
+                                   ((vl.w + (long double)(2.0 * p) * refT) * gl.nJ + cross(vecof(vl), gl.nJ));
+        out4[4 * q] = (double)vl.x; out4[4 * q + 1] = (double)vl.y; out4[4 * q + 2] = (double)vl.z; out4[4 * q + 3] = (double)vl.w;
+        out3[3 * q] = (double)Jl.x; out3[3 * q + 1] = (double)Jl.y; out3[3 * q + 2] = (double)Jl.z;
     }
+}
+void o_tasks_at_levels(void* h, int type, const int32_t* ij, const int8_t* rlevel, int64_t n, const double* L12,
+                       const double* w, int nPts, int order, double* out4, double* out3) {
+    o_tasks_at_levels_prec(h, type, ij, rlevel, n, L12, w, nPts, order, 0, out4, out3);
 }
 
 // Wall-clock helper for the CPU baseline.

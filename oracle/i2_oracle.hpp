@@ -87,6 +87,50 @@ template <class R> struct DualT {
 using Dual = DualT<double>;
 using DualL = DualT<long double>;
 
+// ---------------------------------------------------------------- Monte Carlo arithmetic
+// FP64 value whose every operation and library call is followed by a random
+// perturbation of 0 or +-1 ulp (probabilities 1/2, 1/4, 1/4).  The spread of
+// a formula over a few seeds is its rounding noise: where the reference's
+// closed forms are ill-conditioned, two correct FP64 evaluations (different
+// compilers or math libraries) differ by about that much.
+inline uint64_t& mca_state() { static thread_local uint64_t s = 0; return s; }
+inline double mca(double x) {
+    uint64_t z = (mca_state() += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const int k = (int)(z & 3);
+    if (!std::isfinite(x) || k < 2) return x;
+    return std::nextafter(x, k == 2 ? HUGE_VAL : -HUGE_VAL);
+}
+struct MCA {
+    double v = 0.0;
+    MCA() = default;
+    MCA(double x) : v(x) {}
+    friend MCA operator+(MCA a, MCA b) { return mca(a.v + b.v); }
+    friend MCA operator-(MCA a, MCA b) { return mca(a.v - b.v); }
+    friend MCA operator-(MCA a) { return MCA(-a.v); }
+    friend MCA operator*(MCA a, MCA b) { return mca(a.v * b.v); }
+    friend MCA operator/(MCA a, MCA b) { return mca(a.v / b.v); }
+    friend MCA& operator+=(MCA& a, MCA b) { a = a + b; return a; }
+    friend MCA& operator-=(MCA& a, MCA b) { a = a - b; return a; }
+    friend MCA& operator*=(MCA& a, MCA b) { a = a * b; return a; }
+    friend MCA& operator/=(MCA& a, MCA b) { a = a / b; return a; }
+    friend bool operator<(MCA a, MCA b) { return a.v < b.v; }
+    friend bool operator>(MCA a, MCA b) { return a.v > b.v; }
+    friend bool operator<=(MCA a, MCA b) { return a.v <= b.v; }
+    friend bool operator>=(MCA a, MCA b) { return a.v >= b.v; }
+    friend MCA sqrt(MCA a) { return mca(std::sqrt(a.v)); }
+    friend MCA log(MCA a) { return mca(std::log(a.v)); }
+    friend MCA sin(MCA a) { return mca(std::sin(a.v)); }
+    friend MCA cos(MCA a) { return mca(std::cos(a.v)); }
+    friend MCA tan(MCA a) { return mca(std::tan(a.v)); }
+    friend MCA acos(MCA a) { return mca(std::acos(a.v)); }
+    friend MCA atan2(MCA y, MCA x) { return mca(std::atan2(y.v, x.v)); }
+    friend MCA fabs(MCA a) { return MCA(std::fabs(a.v)); }
+    friend double val(const MCA& x) { return x.v; }
+};
+
 inline double val(double x) { return x; }
 inline long double val(long double x) { return x; }
 using std::sqrt; using std::log; using std::sin; using std::cos; using std::tan;

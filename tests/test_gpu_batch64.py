@@ -44,9 +44,16 @@ def oracle_rows(orc, i2, nT, single, label, maxLevel=3, tol=1e-8):
                                              want_levels=True)
         Ks.append(K)
         conv_gpu = single.pairResults(0, t)["conv"]
-        check_decisions(conv_gpu, conv, lev, tol, f"{label} type {t}")
+        flips = check_decisions(conv_gpu, conv, lev, tol, f"{label} type {t}",
+                                referee=lambda idx: orc.run_tasks(t, tasks[idx], i2.QF3D13_L12, i2.QF3D13_W, 7, mode=1,
+                                                                  level=maxLevel, tol=tol, want_levels=True,
+                                                                  long_double=True)[2::2])
         rl = reported_levels(conv_gpu, single.refinementIterations(0, t), True)
         I4g, J3g = orc.values_at_levels(t, tasks, rl, i2.QF3D13_L12, i2.QF3D13_W, 7)
+        if len(flips):
+            # the FP64 values are not a valid referee where the decisions flip
+            I4g[flips], J3g[flips] = orc.values_at_levels(t, tasks[flips], rl[flips], i2.QF3D13_L12, i2.QF3D13_W, 7,
+                                                          long_double=True)
         add_rows(rows, absrows, tasks[:, 0], np.concatenate([J3g, I4g[:, 3:4]], axis=1))
     return rows, absrows, Ks
 

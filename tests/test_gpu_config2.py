@@ -1,0 +1,105 @@
+"""Parity at full size: the jittered ellipsoid (3, 1, 0.5), icosphere-4
+(5,120 triangles, 26,209,280 ordered pairs), automatic refinement up to 3
+levels at tolerance 1e-8, both integration paths.
+
+- Every vertex- and edge-sharing pair (61,380) and a seeded sample of 20,000
+  disjoint pairs against the oracle, with the GPU run's K imposed: identical
+  refinement decisions except where the Runge criterion lies within its
+  rounding noise of the tolerance, and 1e-11 norm-wise (noise-aware) on the
+  values at the GPU's reported levels.
+- The plain and batched paths agree on every pair (K, decisions up to the same
+  borderline band, values).
+- The symmetry delta of runAllPairs(checkCorrectness) equals the reference's
+  formula applied to the returned J."""
+import numpy as np
+import pytest
+
+from tests.helpers import TOL, check_decisions, config2_mesh, parity_gate, reported_levels
+from tests.test_gpu_parity import run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+def ld_referee(orc, gpu, t, tasks, K):
+    """Decisions and level values of `tasks` taken on long-double level values."""
+    def run(idx):
+        _, _, conv, _, lev = orc.run_tasks(t, tasks[idx], gpu.QF3D13_L12, gpu.QF3D13_W, 7, mode=1, level=3, tol=1e-8,
+                                           K_override=K, want_levels=True, long_double=True)
+        return conv, lev
+    return run
+
+
+@pytest.fixture(scope="module")
+def runs(gpu):
+    xyz, tri = config2_mesh(gpu)
+    out = {}
+    for path in (1, 0):
+        print(f"config 2: path {path}", flush=True)
+        _, res, Ks = run_gpu(gpu, xyz, tri, maxLevel=3, tol=1e-8, quirk=True, path=path, check=True)
+        out[path] = (res, Ks)
+    return xyz, tri, out
+
+
+def test_config2_against_oracle(gpu, oracle_mod, runs):
+    xyz, tri, out = runs
+    orc = oracle_mod.Oracle(xyz, tri)
+    res, Ks = out[1]
+    rng = np.random.default_rng(0)
+    for t in range(3):
+        r = res[t]
+        n = len(r["tasks"])
+        idx = np.arange(n) if t < 2 else np.sort(rng.choice(n, size=20000, replace=False))
+        tasks = r["tasks"][idx]
+        print(f"type {t}: {len(idx)} of {n} pairs against the oracle", flush=True)
+        I4, J3, conv, K, lev = orc.run_tasks(t, tasks, gpu.QF3D13_L12, gpu.QF3D13_W, 7, mode=1, level=3, tol=1e-8,
+                                             K_override=Ks[t], want_levels=True)
+        flips = check_decisions(r["conv"][idx], conv, lev, 1e-8, f"config 2 type {t}", referee=ld_referee(orc, gpu, t, tasks, Ks[t]))
+        rl = reported_levels(r["conv"][idx], Ks[t], True)
+        I4g, J3g = orc.values_at_levels(t, tasks, rl, gpu.QF3D13_L12, gpu.QF3D13_W, 7)
+        if len(flips):
+            I4g[flips], J3g[flips] = orc.values_at_levels(t, tasks[flips], rl[flips], gpu.QF3D13_L12, gpu.QF3D13_W, 7,
+                                                          long_double=True)
+        parity_gate(orc, gpu, t, tasks, r["integrals"][idx], I4g, rl, f"config 2 type {t} integrals")
+        parity_gate(orc, gpu, t, tasks, r["results"][idx], J3g, rl, f"config 2 type {t} results", results=True)
+
+
+def test_config2_paths_agree(gpu, oracle_mod, runs):
+    xyz, tri, out = runs
+    (r1, K1), (r0, K0) = out[1], out[0]
+    assert K1 == K0
+    orc = oracle_mod.Oracle(xyz, tri)
+    for t in range(3):
+        a, b = r1[t], r0[t]
+        assert np.array_equal(a["tasks"], b["tasks"])
+        diff = np.nonzero(a["conv"] != b["conv"])[0]
+        same = np.setdiff1d(np.arange(len(a["conv"])), diff)
+        print(f"type {t}: {len(diff)} of {len(same) + len(diff)} decisions differ between the paths", flush=True)
+        if len(diff):
+            _, _, conv, _, lev = orc.run_tasks(t, a["tasks"][diff], gpu.QF3D13_L12, gpu.QF3D13_W, 7, mode=1, level=3,
+                                               tol=1e-8, K_override=K1[t], want_levels=True)
+            # each path's differing decision equals the long-double one or is borderline
+            for path_conv in (a["conv"][diff], b["conv"][diff]):
+                m = np.nonzero(path_conv != conv)[0]
+                if len(m):
+                    check_decisions(path_conv[m], conv[m], lev[m], 1e-8, f"paths type {t}", max_frac=1.0,
+                                    referee=ld_referee(orc, gpu, t, a["tasks"][diff][m], K1[t]))
+        rl = reported_levels(a["conv"][same], K1[t], True)
+        e = np.abs(a["integrals"][same] - b["integrals"][same]).max(axis=1) / np.abs(b["integrals"][same]).max(axis=1)
+        bad = np.nonzero(e > TOL)[0]
+        if len(bad):
+            # both paths are within the noise-aware gate of the oracle at those pairs
+            sel = same[bad[:2000]]
+            I4g, _ = orc.values_at_levels(t, a["tasks"][sel], rl[bad[:2000]], gpu.QF3D13_L12, gpu.QF3D13_W, 7)
+            parity_gate(orc, gpu, t, a["tasks"][sel], a["integrals"][sel], I4g, rl[bad[:2000]], f"batched type {t}")
+            parity_gate(orc, gpu, t, b["tasks"][sel], b["integrals"][sel], I4g, rl[bad[:2000]], f"plain type {t}")
+        print(f"type {t}: paths differ above 1e-11 at {len(bad)} pairs (max {e.max():.2e})", flush=True)
+
+
+def test_config2_symmetry_delta(gpu, oracle_mod, runs):
+    _, _, out = runs
+    for path in (0, 1):
+        res, _ = out[path]
+        for t in range(3):
+            r = res[t]
+            ref = oracle_mod.symmetry(r["results"])
+            assert np.abs(r["errors"] - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (path, t)

@@ -1,0 +1,39 @@
+"""The kernels' FP64 sqrt, log and atan2 on the device (hardware 1/sqrt
+estimate, LDS log table) within 1-2 ulp of 120-bit references, like their
+host builds (test_cpu_library)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.test_cpu_library import _ulp_err
+
+pytestmark = pytest.mark.gpu
+
+
+def device_math(i2, fn, x, y=None):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    P = ctypes.c_void_p
+    yp = None if y is None else np.ascontiguousarray(y, np.float64).ctypes.data_as(P)
+    st = i2.lib().i2_device_math(ctypes.c_int(0), ctypes.c_int(fn), x.ctypes.data_as(P), yp, ctypes.c_int64(len(x)),
+                                 out.ctypes.data_as(P))
+    assert st == 0, i2.lib().i2_last_error()
+    return out
+
+
+def test_device_sqrt_log_atan2(gpu):
+    import mpmath
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(5)
+    x = np.concatenate([np.exp(rng.uniform(-600, 600, 3000)), 1.0 + rng.uniform(-1e-3, 1e-3, 1000)])
+    e_sqrt = _ulp_err(device_math(gpu, 2, x), [mpmath.sqrt(mpmath.mpf(v)) for v in x])
+    e_log = _ulp_err(device_math(gpu, 0, x), [mpmath.log(mpmath.mpf(v)) for v in x])
+    y = rng.normal(size=3000) * np.exp(rng.uniform(-20, 20, 3000))
+    xx = rng.normal(size=3000) * np.exp(rng.uniform(-20, 20, 3000))
+    e_atan = _ulp_err(device_math(gpu, 1, xx, y), [mpmath.atan2(mpmath.mpf(a), mpmath.mpf(b)) for a, b in zip(y, xx)])
+    lib_sqrt = _ulp_err(device_math(gpu, 5, x), [mpmath.sqrt(mpmath.mpf(v)) for v in x])
+    lib_log = _ulp_err(device_math(gpu, 3, x), [mpmath.log(mpmath.mpf(v)) for v in x])
+    print(f"device ulp errors: sqrt {e_sqrt.max():.2f} (library {lib_sqrt.max():.2f}), log {e_log.max():.2f} "
+          f"(library {lib_log.max():.2f}), atan2 {e_atan.max():.2f}", flush=True)
+    assert e_sqrt.max() <= 1.0 and e_log.max() <= 2.0 and e_atan.max() <= 2.0
