@@ -238,7 +238,7 @@ __device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS&
 // Disjoint pairs, levels 0 and 1: item = (mesh, control panel i, P consecutive j),
 // processed in rounds of blockDim/LPP pairs.
 template <int OUT, int LPP>
-__global__ __launch_bounds__(256) void k_sweep_disjoint(BatchArgs A) {
+__global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_disjoint(BatchArgs A) {
     __shared__ RuleLDS R;
     __shared__ long long item_sh;
     load_rule(R, A.rule);
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_sweep_disjoint(BatchArgs A) {
 
 // Vertex- and edge-sharing tasks, levels 0 and 1: one task per lane, 256 per item.
 template <int OUT>
-__global__ __launch_bounds__(256) void k_sweep_singular(BatchArgs A) {
+__global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_singular(BatchArgs A) {
     __shared__ RuleLDS R;
     __shared__ long long item_sh;
     load_rule(R, A.rule);
@@ -368,7 +368,7 @@ __device__ __forceinline__ void deep_task(const BatchArgs& A, const RuleLDS& R, 
 
 // Refinement level L >= 2 of the queued tasks of the types in [t0, t1].
 template <int OUT, int T0, int T1>
-__global__ __launch_bounds__(256) void k_deep(BatchArgs A, int level) {
+__global__ __launch_bounds__(256) I2_TWO_WAVES void k_deep(BatchArgs A, int level) {
     __shared__ RuleLDS R;
     load_rule(R, A.rule);
     __syncthreads();

@@ -14,6 +14,10 @@
 // Device math is also compiled for the host so that the C ABI can expose
 // host-side evaluations of the same formulas for checking.
 #define I2_HD __host__ __device__
+// Hot FP64 kernels: cap the unified VGPR+AGPR budget at 256 so that two
+// wavefronts fit per SIMD; the quadrature loop's latency (LDS table loads,
+// transcendental chains) needs the second wavefront to hide it.
+#define I2_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2)))
 
 #include "i2_math.hpp"
 
