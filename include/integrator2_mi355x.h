@@ -53,12 +53,14 @@ int i2_device_count(int* n);
  * fast = 1 the division-free form used by the batched kernels.  For checking. */
 int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9, double* out4);
 /* Host-side evaluation of the kernels' FP64 logarithm (fn = 0, of x),
- * two-argument arctangent (fn = 1, atan2(y, x)) and square root (fn = 2, of
- * x; the host uses the library's 1/sqrt as the estimate the device takes
- * from the hardware).  For checking. */
+ * two-argument arctangent (fn = 1, atan2(y, x)), square root (fn = 2, of x;
+ * the host uses the library's 1/sqrt as the estimate the device takes from
+ * the hardware) and reciprocal (fn = 6, of x; the host starts from a
+ * single-precision estimate).  For checking. */
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out);
-/* The same functions evaluated on the device (fn 0-2 as above; 3, 4, 5: the
- * HIP library's log, atan2 and sqrt, for comparison).  For checking. */
+/* The same functions evaluated on the device (fn 0-2 and 6 as above; 3, 4,
+ * 5, 7: the HIP library's log, atan2, sqrt and 1/x, for comparison).  For
+ * checking. */
 int i2_device_math(int device, int fn, const double* x, const double* y, int64_t n, double* out);
 /* Host-side evaluation of one ordered task's integrals (Psi, Theta) at
  * refinement level `level` with the kernels' device code compiled for the

@@ -104,13 +104,13 @@ __device__ __forceinline__ double dot4(q4 a, q4 b) { return a.x * b.x + a.y * b.
 
 struct RuleSh {
     double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
-    i2m::LogEntry lt[i2m::LOGTAB_N];
+    i2m::LogTable lt;
 };
 __device__ __forceinline__ void load_rule_sh(RuleSh& s, const Rule& R) {
     for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
         s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
     }
-    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) s.lt[k] = i2m::c_logtab[k];
+    i2m::fill_logtab(s.lt);
 }
 
 // Vertex cotangents of a panel from those of its tangents and normal
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_disjoint(AdjArgs A) {
                     p = p + R.L2[k] * b;
                     p = p + R.L3[k] * c;
                     d3 pb;
-                    const q4 f = theta_psi_fast_rev(p, JG, R.lt, (fa * R.w[k]) * gb, &pb, g);
+                    const q4 f = theta_psi_fast_rev(p, JG, &R.lt, (fa * R.w[k]) * gb, &pb, g);
                     F = F + R.w[k] * f;
                     axpy(ab, R.L1[k], pb); axpy(bbar, R.L2[k], pb); axpy(cb, R.L3[k], pb);
                 }
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_deep(AdjArgs A) {
             p = p + R.L2[k] * b;
             p = p + R.L3[k] * c;
             d3 pb;
-            const q4 f = theta_psi_fast_rev(p, JG, R.lt, (fa * R.w[k]) * gb, &pb, g);
+            const q4 f = theta_psi_fast_rev(p, JG, &R.lt, (fa * R.w[k]) * gb, &pb, g);
             F = F + R.w[k] * f;
             const d3 beta = R.L1[k] * ba + R.L2[k] * bb + R.L3[k] * bc;
             axpy(X0, beta.x, pb); axpy(X1, beta.y, pb); axpy(X2, beta.z, pb);
@@ -384,7 +384,7 @@ __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R,
         const q4 gp = (fa * R.w[k]) * gb;
         const q4 gm = (-1.0) * gp;
         d3 pb, ps;
-        q4 f = theta_psi_fast_rev(p, JG, R.lt, gp, &pb, B.g);
+        q4 f = theta_psi_fast_rev(p, JG, &R.lt, gp, &pb, B.g);
         if (TYPE == I2_ATTACHED_NEIGHBORS) f = f - sing_edge_rev(p, ec, gm, &ps, B.ge);
         else f = f - sing_vertex_rev(p, vc, gm, &ps, B.gv);
         pb = pb + ps;
@@ -679,7 +679,7 @@ int i2_host_point_adjoint(int kind, const double* p3, const double* params, cons
     if (kind == 0) {
         const JGeom J = {v(0), v(3), v(6), v(9), v(12), v(15)};
         TPBar g = {z3, z3, z3, z3, z3, z3};
-        f = theta_psi_fast_rev(p, J, i2m::h_logtab, gb, &pb, g);
+        f = theta_psi_fast_rev(p, J, &i2m::h_logtab, gb, &pb, g);
         put(0, g.a); put(3, g.b); put(6, g.c); put(9, g.ta); put(12, g.tb); put(15, g.tc);
     } else if (kind == 1) {
         const EdgeCtx c = {v(0), v(3), v(6), v(9), v(12), params[15]};

@@ -18,7 +18,7 @@ I2_HD __forceinline__ void axpy(d3& y, double a, d3 x) { y.x += a * x.x; y.y += 
 // cotangents of the parameters of thetaPsi over K_j: vertices and edge tangents
 struct TPBar { d3 a, b, c, ta, tb, tc; };
 
-I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::LogEntry* __restrict__ lt, q4 gb, d3* pbar,
+I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::LogTable* __restrict__ lt, q4 gb, d3* pbar,
                                             TPBar& g) {
     // ---- forward, as theta_psi_fast
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
@@ -27,7 +27,7 @@ I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::Log
     bool f1, f2, f3;
     theta_psi_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
     const double e12 = e1 * e2;
-    const double inv = 1.0 / (e12 * e3);
+    const double inv = i2m::rcp_pos(e12 * e3);
     const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);
     const double t2 = i2m::log_tab(n2 * (e1 * e3) * inv, lt);
     const double t3 = i2m::log_tab(n3 * e12 * inv, lt);

@@ -105,14 +105,14 @@ __device__ __noinline__ q4 singular_integral(int type, const Panel& I, const Pan
 // Rule held in LDS: (L1, L2, L3, w) per point, and the log reduction table
 struct RuleLDS {
     double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
-    i2m::LogEntry lt[i2m::LOGTAB_N];
+    i2m::LogTable lt;
 };
 
 __device__ __forceinline__ void load_rule(RuleLDS& s, const Rule& R) {
     for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
         s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
     }
-    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) s.lt[k] = i2m::c_logtab[k];
+    i2m::fill_logtab(s.lt);
 }
 
 // Partial quadrature sum_k w_k f(r_k) of one sub-panel over points k = s, s + step, ...
@@ -124,9 +124,41 @@ __device__ __forceinline__ q4 quad_part(const RuleLDS& R, int npts, int s, int s
         d3 p = R.L1[k] * a;
         p = p + R.L2[k] * b;
         p = p + R.L3[k] * c;
-        q4 f = theta_psi_fast(p, J, R.lt);
+        q4 f = theta_psi_fast(p, J, &R.lt);
         if (TYPE != I2_NOT_NEIGHBORS) f = f - sg.at(p);
         acc = acc + R.w[k] * f;
+    }
+    return acc;
+}
+
+// Quadrature points of a disjoint item's control panel K_i (uniform over the
+// workgroup): the level-0 panel and its four level-1 children, in the order
+// sweep_task sums them, computed once per item instead of once per pair.
+struct PtsLDS { double x[5 * MAX_POINTS], y[5 * MAX_POINTS], z[5 * MAX_POINTS]; };
+
+__device__ __forceinline__ void load_points(PtsLDS& P, const RuleLDS& R, int npts, const Panel& I) {
+    const d3 mBC = 0.5 * (I.b + I.c), mCA = 0.5 * (I.c + I.a), mAB = 0.5 * (I.a + I.b);
+    for (int t = threadIdx.x; t < 5 * npts; t += blockDim.x) {
+        const int sub = t / npts, k = t - sub * npts;
+        d3 a = I.a, b = I.b, c = I.c;
+        if (sub == 1) { a = mAB; c = mBC; }
+        else if (sub == 2) { a = mBC; b = I.c; c = mCA; }
+        else if (sub == 3) { a = mCA; b = I.a; c = mAB; }
+        else if (sub == 4) { a = mBC; b = mCA; c = mAB; }
+        d3 p = R.L1[k] * a;
+        p = p + R.L2[k] * b;
+        p = p + R.L3[k] * c;
+        P.x[t] = p.x; P.y[t] = p.y; P.z[t] = p.z;
+    }
+}
+
+// Partial quadrature sum over the precomputed points base + k, k = s, s + step, ...
+__device__ __forceinline__ q4 quad_pts(const RuleLDS& R, const PtsLDS& P, int base, int npts, int s, int step,
+                                       const JGeom& J) {
+    q4 acc = q4zero();
+    for (int k = s; k < npts; k += step) {
+        const d3 p{P.x[base + k], P.y[base + k], P.z[base + k]};
+        acc = acc + R.w[k] * theta_psi_fast(p, J, &R.lt);
     }
     return acc;
 }
@@ -198,7 +230,7 @@ __device__ __forceinline__ void add_row(const MeshDesc& M, int type, int i, q4 r
 // Levels 0 and 1 of one task (lane group of LPP lanes), then the Runge check.
 template <int TYPE, int LPP>
 __device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS& R, int m, int i, int j, int64_t z, bool valid,
-                                              int s, const Panel& I, const Panel& J) {
+                                              int s, const Panel& I, const Panel& J, const PtsLDS* pts = nullptr) {
     Sing<TYPE> sg;
     if (TYPE != I2_NOT_NEIGHBORS && valid) sg.init(I, J);
     const JGeom JG = jgeom(J);
@@ -206,7 +238,15 @@ __device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS&
     const bool need0 = !A.fixed || A.last_level == 0;
     const bool need1 = A.fixed ? A.last_level == 1 : A.last_level >= 1;
     q4 v0 = q4zero(), v1 = q4zero();
-    if (valid) {
+    if constexpr (TYPE == I2_NOT_NEIGHBORS) {
+        if (valid) {
+            if (need0) v0 = I.area * quad_pts(R, *pts, 0, npts, s, LPP, JG);
+            if (need1) {
+                const double a1 = 0.25 * I.area;
+                for (int sub = 1; sub <= 4; ++sub) v1 = v1 + a1 * quad_pts(R, *pts, sub * npts, npts, s, LPP, JG);
+            }
+        }
+    } else if (valid) {
         if (need0) v0 = I.area * quad_part<TYPE>(R, npts, s, LPP, I.a, I.b, I.c, JG, sg);
         if (need1) {
             const double a1 = 0.25 * I.area;
@@ -240,6 +280,7 @@ __device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS&
 template <int OUT, int LPP>
 __global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_disjoint(BatchArgs A) {
     __shared__ RuleLDS R;
+    __shared__ PtsLDS pts;
     __shared__ long long item_sh;
     load_rule(R, A.rule);
     const int P = A.pairs_per_item;
@@ -257,17 +298,16 @@ __global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_disjoint(BatchArgs A
         const int tiles = (M.nT + P - 1) / P;
         const int i = __builtin_amdgcn_readfirstlane((int)(r / tiles));
         const Panel I = M.panels[i];
+        load_points(pts, R, A.rule.n, I);
+        __syncthreads();
         for (int round = 0; round < P; round += per_round) {
         const int p = round + p0;
         const int j = (int)(r % tiles) * P + p;
-        bool valid = p < P && j < M.nT && j != i;
-        Panel J = I;
-        if (valid) {
-            J = M.panels[j];
-            valid = shared_count(I, J) == 0;
-        }
+        const bool in_range = p < P && j < M.nT;
+        const Panel J = M.panels[in_range ? j : i];
+        const bool valid = in_range && j != i && shared_count(I, J) == 0;
         const int64_t z = valid ? disjoint_task(M, i, j) : 0;
-        const StepOut o = sweep_task<I2_NOT_NEIGHBORS, LPP>(A, R, m, i, j, z, valid, s, I, J);
+        const StepOut o = sweep_task<I2_NOT_NEIGHBORS, LPP>(A, R, m, i, j, z, valid, s, I, J, &pts);
         q4 rp0 = q4zero(), rp1 = q4zero();
         if (o.finished) finish_task<I2_NOT_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
         if (OUT == OUT_ROW_SUMS) {

@@ -133,8 +133,8 @@ I2_HD __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, 
 // d = p - vertex and l = |d|, the reference's l (1 + (d/l).tau) is l + d.tau,
 // and the solid-angle atan2 has both arguments multiplied by la lb lc > 0
 // (Van Oosterom and Strackee, IEEE Trans. Biomed. Eng. 30 (1983) 125-126):
-// identical in exact arithmetic, with 3 square roots, 1 division, 3 logs and
-// 1 atan2 per point instead of 13 divisions (sqrt, log and atan2 from
+// identical in exact arithmetic, with 3 square roots, 1 reciprocal, 3 logs
+// and 1 atan2 per point instead of 13 divisions (sqrt, log and atan2 from
 // i2_math.hpp, within 2 ulp; `lt` is the log reduction table, in LDS on the
 // device).  The degenerate-branch tests |1 + (d/l).tau| < eps become
 // |l + d.tau| < eps l.
@@ -159,15 +159,17 @@ I2_HD __forceinline__ void theta_psi_terms(const JGeom& J, d3 da, d3 db, d3 dc, 
     n2 = lb + dot(db, J.ta); e2 = lc + dot(dc, J.ta);
     n3 = lc + dot(dc, J.tb); e3 = la + dot(da, J.tb);
     const double s = 1e-4;
-    const bool c1 = n1 < s * la, c2 = e1 < s * lb, c3 = n2 < s * lb, c4 = e2 < s * lc, c5 = n3 < s * lc, c6 = e3 < s * la;
     f1 = f2 = f3 = false;
-    if (__builtin_expect(c1 | c2 | c3 | c4 | c5 | c6, 0)) {
-        if (c1) n1 = l_plus_dtau(la, da, J.tc);
-        if (c2) e1 = l_plus_dtau(lb, db, J.tc);
-        if (c3) n2 = l_plus_dtau(lb, db, J.ta);
-        if (c4) e2 = l_plus_dtau(lc, dc, J.ta);
-        if (c5) n3 = l_plus_dtau(lc, dc, J.tb);
-        if (c6) e3 = l_plus_dtau(la, da, J.tb);
+    // one test per distance (the per-term tests are repeated inside the rare
+    // branch rather than kept live through the common path)
+    const double m = fmin(fmin(fmin(n1, e3) - s * la, fmin(e1, n2) - s * lb), fmin(e2, n3) - s * lc);
+    if (__builtin_expect(m < 0.0, 0)) {
+        if (n1 < s * la) n1 = l_plus_dtau(la, da, J.tc);
+        if (e1 < s * lb) e1 = l_plus_dtau(lb, db, J.tc);
+        if (n2 < s * lb) n2 = l_plus_dtau(lb, db, J.ta);
+        if (e2 < s * lc) e2 = l_plus_dtau(lc, dc, J.ta);
+        if (n3 < s * lc) n3 = l_plus_dtau(lc, dc, J.tb);
+        if (e3 < s * la) e3 = l_plus_dtau(la, da, J.tb);
         // a point on an edge line of K_j keeps the ratio of distances (evaluatorJ3DK.cu:294-307)
         const double h = 0.5 * EPS_PSI_THETA2;
         f1 = fabs(e1) < h * lb; f2 = fabs(e2) < h * lc; f3 = fabs(e3) < h * la;
@@ -176,14 +178,14 @@ I2_HD __forceinline__ void theta_psi_terms(const JGeom& J, d3 da, d3 db, d3 dc, 
         if (f3) { n3 = la; e3 = lc; }
     }
 }
-I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogEntry* __restrict__ lt) {
+I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogTable* __restrict__ lt) {
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
     const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
     double n1, e1, n2, e2, n3, e3;
     bool f1, f2, f3;
     theta_psi_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
     const double e12 = e1 * e2;
-    const double inv = 1.0 / (e12 * e3);
+    const double inv = i2m::rcp_pos(e12 * e3);
     const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);
     const double t2 = i2m::log_tab(n2 * (e1 * e3) * inv, lt);
     const double t3 = i2m::log_tab(n3 * e12 * inv, lt);

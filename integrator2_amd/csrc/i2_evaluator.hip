@@ -81,7 +81,7 @@ template <> struct PairCtx<I2_SIMPLE_NEIGHBORS> {
 // quadrature point lies near the extension of an edge line of K_j, which
 // moves Runge decisions; both paths therefore take decisions on accurate values.
 template <int TYPE>
-__device__ __forceinline__ q4 quad_sub(const Rule& R, const i2m::LogEntry* lt, d3 a, d3 b, d3 c, double area,
+__device__ __forceinline__ q4 quad_sub(const Rule& R, const i2m::LogTable* lt, d3 a, d3 b, d3 c, double area,
                                        const JGeom& J, const PairCtx<TYPE>& ctx) {
     q4 acc = q4zero();
     for (int k = 0; k < R.n; ++k) {
@@ -101,8 +101,8 @@ template <int TYPE, bool IMPLICIT>
 __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __restrict__ list, TaskSource src,
                                                const Panel* __restrict__ panels, Rule R, int level, int lanes_log2,
                                                double4* __restrict__ out) {
-    __shared__ i2m::LogEntry lt[i2m::LOGTAB_N];
-    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) lt[k] = i2m::c_logtab[k];
+    __shared__ i2m::LogTable lt;
+    i2m::fill_logtab(lt);
     __syncthreads();
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int G = 1 << lanes_log2;
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __
         for (uint32_t c = sub; c < nsub; c += G) {
             d3 a = I.a, b = I.b, cc = I.c;
             sub_panel(a, b, cc, level, c);
-            acc = acc + quad_sub<TYPE>(R, lt, a, b, cc, area, JG, ctx);
+            acc = acc + quad_sub<TYPE>(R, &lt, a, b, cc, area, JG, ctx);
         }
     }
     for (int m = 1; m < G; m <<= 1) {

@@ -24,7 +24,11 @@
 //          atan t = atan c + atan u, u = (n - c d)/(d + c n) for t = n/d, so
 //          |u| <= tan(pi/16); atan u by its Taylor series to u^21/21;
 //          then the octant and quadrant identities.
-// Non-finite, zero, negative and subnormal arguments take the library path.
+//   1/x and a/b (rcp_pos, div_pos): hardware reciprocal estimate refined by
+//          Newton steps with fma residuals; used where the operand is a
+//          product of distances, far inside [2^-1000, 2^1000].
+// Non-finite, zero, negative and subnormal arguments take the library path
+// (log, atan2).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -56,6 +60,40 @@ static I2_HD __noinline__ double atan2_slow(double y, double x) { return ::atan2
 __device__ __forceinline__ double rsq_estimate(double s) { return __builtin_amdgcn_rsq(s); }
 __host__ inline double rsq_estimate(double s) { return 1.0 / ::sqrt(s); }
 
+// estimate of 1/x: the hardware instruction on the device; on the host a
+// single-precision reciprocal of the mantissa, so that host checks exercise
+// the refinement from a low-precision start
+__device__ __forceinline__ double rcp_estimate(double x) { return __builtin_amdgcn_rcp(x); }
+__host__ inline double rcp_estimate(double x) {
+    int k;
+    const double m = ::frexp(x, &k);
+    return ::ldexp((double)(1.0f / (float)m), -k);
+}
+
+// 1/x for |x| in [2^-1000, 2^1000]: estimate, then three Newton steps
+// y += y (1 - x y), each with the residual exact in one fma
+I2_HD __forceinline__ double rcp_pos(double x) {
+    double y = rcp_estimate(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// a / b for |b| in [2^-1000, 2^1000]: two Newton steps on 1/b, then the
+// quotient with one residual correction
+I2_HD __forceinline__ double div_pos(double a, double b) {
+    double y = rcp_estimate(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+}
+
 // sqrt of a positive normal number (squared distances of the hot loop)
 I2_HD __forceinline__ double sqrt_pos(double s) {
     const double y = rsq_estimate(s);
@@ -67,63 +105,91 @@ I2_HD __forceinline__ double sqrt_pos(double s) {
     return fma(d, h, g);
 }
 
-// log reduction table (gen_log_table.py): entry j + LOGTAB_J0 for c_j = 1 + j/64
+// a * b + c for a Horner step whose addend c is a constant.  On the device
+// this is one three-operand v_fma_f64: left to itself the compiler keeps such
+// constants in VGPRs and emits a copy into the accumulator before each
+// two-operand v_fmac_f64.
+I2_HD __forceinline__ double fma_k(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+#else
+    return fma(a, b, c);
+#endif
+}
+
+// log reduction table (gen_log_table.py): entry j + LOGTAB_J0 for c_j = 1 + j/64,
+// held as three arrays so that an entry's address is a shift of j
 struct LogEntry { double invc, thi, tlo; };
+struct LogTable { double invc[47], thi[47], tlo[47]; };
 constexpr int LOGTAB_J0 = 19;
 constexpr int LOGTAB_N = 47;
-static const LogEntry h_logtab[LOGTAB_N] = {
+constexpr LogEntry h_logent[LOGTAB_N] = {
 #include "i2_logtab.inc"
 };
+constexpr LogTable make_logtab(const LogEntry (&e)[LOGTAB_N]) {
+    LogTable t{};
+    for (int k = 0; k < LOGTAB_N; ++k) { t.invc[k] = e[k].invc; t.thi[k] = e[k].thi; t.tlo[k] = e[k].tlo; }
+    return t;
+}
+static constexpr LogTable h_logtab = make_logtab(h_logent);
 static __constant__ LogEntry c_logtab[LOGTAB_N] = {
 #include "i2_logtab.inc"
 };
+// copies the table into `s` (LDS) with the threads of the block; the caller synchronises
+__device__ __forceinline__ void fill_logtab(LogTable& s) {
+    for (int k = threadIdx.x; k < LOGTAB_N; k += blockDim.x) {
+        s.invc[k] = c_logtab[k].invc; s.thi[k] = c_logtab[k].thi; s.tlo[k] = c_logtab[k].tlo;
+    }
+}
 
 // log of a positive normal number with the reduction table `tab` (LDS on the device)
-I2_HD __forceinline__ double log_tab(double x, const LogEntry* __restrict__ tab) {
+I2_HD __forceinline__ double log_tab(double x, const LogTable* __restrict__ tab) {
     if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log_slow(x);
     int k;
     double m = frexp(x, &k);             // x = m 2^k, m in [0.5, 1)
     if (m < SQRT_HALF) { m += m; --k; }  // m in [sqrt(1/2), sqrt(2))
-    const int j = (int)rint(fma(m, 64.0, -64.0));
-    const LogEntry e = tab[j + LOGTAB_J0];
-    const double r = fma(m, e.invc, -1.0);
+    // nearest centre (ties upward): 64 m + 1/2 in [45.7, 91.1), so j in [-19, 27]
+    const int j = (int)fma(m, 64.0, 0.5) - 64 + LOGTAB_J0;
+    const double r = fma(m, tab->invc[j], -1.0);
     // log1p(r) = r - r^2/2 + ... - r^8/8
-    double q = -1.0 / 8.0;
-    q = fma(q, r, 1.0 / 7.0);
-    q = fma(q, r, -1.0 / 6.0);
-    q = fma(q, r, 1.0 / 5.0);
+    double q = fma_k(r, -1.0 / 8.0, 1.0 / 7.0);
+    q = fma_k(q, r, -1.0 / 6.0);
+    q = fma_k(q, r, 1.0 / 5.0);
     q = fma(q, r, -1.0 / 4.0);
-    q = fma(q, r, 1.0 / 3.0);
+    q = fma_k(q, r, 1.0 / 3.0);
     q = fma(q, r, -0.5);
     const double p = fma(r * r, q, r);
     const double dk = (double)k;
-    return fma(dk, LN2_HI, e.thi) + (fma(dk, LN2_LO, e.tlo) + p);
+    return fma(dk, LN2_HI, tab->thi[j]) + (fma(dk, LN2_LO, tab->tlo[j]) + p);
 }
 
 I2_HD __forceinline__ double atan2_fast(double y, double x) {
     const double ax = fabs(x), ay = fabs(y);
-    if (!(ax <= 1.7976931348623157e308 && ay <= 1.7976931348623157e308) || (ax == 0.0 && ay == 0.0)) return atan2_slow(y, x);
+    // zero, tiny, huge, infinite and NaN arguments take the library path
+    const double sum = ax + ay;
+    if (!(sum >= 0x1p-999 && sum <= 0x1p999)) return atan2_slow(y, x);
     const bool swap = ay > ax;
-    const double n = swap ? ax : ay;
-    const double d = swap ? ay : ax;                 // t = n/d in [0, 1]
+    const double n = fmin(ax, ay);
+    const double d = fmax(ax, ay);                   // t = n/d in [0, 1], d >= 2^-1000
     const bool mid = n > T16 * d;
     const bool high = n > T316 * d;
     const double c = high ? 1.0 : (mid ? C8 : 0.0);
     const double bhi = high ? PIO4_HI : (mid ? ATAN_C8_HI : 0.0);
     const double blo = high ? PIO4_LO : (mid ? ATAN_C8_LO : 0.0);
-    const double u = fma(-c, d, n) / fma(c, n, d);   // |u| <= tan(pi/16)
+    const double u = div_pos(fma(-c, d, n), fma(c, n, d));   // |u| <= tan(pi/16)
     const double w = u * u;
     // -1/3 + w/5 - ... + w^10/21
-    double p = 1.0 / 21.0;
-    p = fma(p, w, -1.0 / 19.0);
-    p = fma(p, w, 1.0 / 17.0);
-    p = fma(p, w, -1.0 / 15.0);
-    p = fma(p, w, 1.0 / 13.0);
-    p = fma(p, w, -1.0 / 11.0);
-    p = fma(p, w, 1.0 / 9.0);
-    p = fma(p, w, -1.0 / 7.0);
-    p = fma(p, w, 1.0 / 5.0);
-    p = fma(p, w, -1.0 / 3.0);
+    double p = fma_k(w, 1.0 / 21.0, -1.0 / 19.0);
+    p = fma_k(p, w, 1.0 / 17.0);
+    p = fma_k(p, w, -1.0 / 15.0);
+    p = fma_k(p, w, 1.0 / 13.0);
+    p = fma_k(p, w, -1.0 / 11.0);
+    p = fma_k(p, w, 1.0 / 9.0);
+    p = fma_k(p, w, -1.0 / 7.0);
+    p = fma_k(p, w, 1.0 / 5.0);
+    p = fma_k(p, w, -1.0 / 3.0);
     const double au = fma(u * w, p, u);               // atan u
     double r = bhi + (au + blo);                      // atan t, t in [0, 1]
     if (swap) r = PIO2_HI - r + PIO2_LO;              // atan(1/t) = pi/2 - atan t

@@ -141,9 +141,10 @@ const char* i2_last_error(void) { return g_last_error.c_str(); }
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out) {
     if (!x || !out || n < 0 || (fn == 1 && !y)) { set_error("null argument"); return I2_ERR_INVALID; }
     for (int64_t q = 0; q < n; ++q) {
-        if (fn == 0) out[q] = i2m::log_tab(x[q], i2m::h_logtab);
+        if (fn == 0) out[q] = i2m::log_tab(x[q], &i2m::h_logtab);
         else if (fn == 2) out[q] = i2m::sqrt_pos(x[q]);
         else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
+        else if (fn == 6) out[q] = i2m::rcp_pos(x[q]);
         else { set_error("unknown function %d", fn); return I2_ERR_INVALID; }
     }
     return I2_OK;
@@ -156,7 +157,7 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
     const JGeom J = {A, B, C, unit(C - B), unit(A - C), unit(B - A)};
     for (int64_t q = 0; q < n; ++q) {
         const d3 p = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
-        const q4 r = fast ? theta_psi_fast(p, J, i2m::h_logtab) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
+        const q4 r = fast ? theta_psi_fast(p, J, &i2m::h_logtab) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
         out4[4 * q] = r.x; out4[4 * q + 1] = r.y; out4[4 * q + 2] = r.z; out4[4 * q + 3] = r.w;
     }
     return I2_OK;
@@ -164,17 +165,19 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
 
 // The kernels' math functions on the device, for checking (fn as in i2_host_math).
 __global__ void k_math(int fn, int64_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out) {
-    __shared__ i2m::LogEntry lt[i2m::LOGTAB_N];
-    for (int k = threadIdx.x; k < i2m::LOGTAB_N; k += blockDim.x) lt[k] = i2m::c_logtab[k];
+    __shared__ i2m::LogTable lt;
+    i2m::fill_logtab(lt);
     __syncthreads();
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    if (fn == 0) out[q] = i2m::log_tab(x[q], lt);
+    if (fn == 0) out[q] = i2m::log_tab(x[q], &lt);
     else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
     else if (fn == 2) out[q] = i2m::sqrt_pos(x[q]);
     else if (fn == 3) out[q] = log(x[q]);
     else if (fn == 4) out[q] = atan2(y[q], x[q]);
-    else out[q] = sqrt(x[q]);
+    else if (fn == 5) out[q] = sqrt(x[q]);
+    else if (fn == 6) out[q] = i2m::rcp_pos(x[q]);
+    else out[q] = 1.0 / x[q];
 }
 
 // Host evaluation of one ordered task's integrals with the kernels' device
@@ -184,7 +187,7 @@ __global__ void k_math(int fn, int64_t n, const double* __restrict__ x, const do
 // singular part.  tri9 are the two panels' vertices, vid6 their vertex ids
 // (they decide the neighbour type and the rotations).  For checking.
 int i2_device_math(int device, int fn, const double* x, const double* y, int64_t n, double* out) {
-    if (!x || !out || n <= 0 || fn < 0 || fn > 5 || ((fn == 1 || fn == 4) && !y)) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!x || !out || n <= 0 || fn < 0 || fn > 7 || ((fn == 1 || fn == 4) && !y)) { set_error("invalid argument"); return I2_ERR_INVALID; }
     DeviceGuard g(device);
     if (!g.ok) return I2_ERR_HIP;
     DBuf<double> dx, dy, dout;
@@ -246,7 +249,7 @@ int i2_host_task_value(const double* I9, const double* J9, const int32_t* vid6, 
             d3 p = l1 * a;
             p = p + l2 * b;
             p = p + l3 * cc;
-            q4 f = theta_psi_fast(p, JG, i2m::h_logtab);
+            q4 f = theta_psi_fast(p, JG, &i2m::h_logtab);
             if (shared == 2) f = f - sing_edge(p, ec);
             if (shared == 1) f = f - sing_vertex(p, vc);
             acc = acc + w[k] * f;

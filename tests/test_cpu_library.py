@@ -94,6 +94,25 @@ def test_host_atan2_within_2_ulp(i2):
     assert e.max() <= 2.0, e.max()
     z = _host_math(i2, 1, np.array([1.0, -1.0, 0.0, 0.0]), np.array([0.0, 0.0, 0.0, -0.0]))
     assert z[0] == 0.0 and z[1] == np.pi and z[2] == 0.0 and np.signbit(z[3])
+    # both sides of the fast path's magnitude window [2^-999, 2^999]
+    big = np.array([1e300, -3e300, 2e-300, -7e-301, 1e-320, 1e308])
+    yb = np.array([2e300, 1e299, -1e-300, 5e-301, 3e-321, -1e307])
+    got = _host_math(i2, 1, big, yb)
+    e = _ulp_err(got, [mpmath.atan2(mpmath.mpf(a), mpmath.mpf(b)) for a, b in zip(yb, big)])
+    assert e.max() <= 2.0, e.max()
+
+
+def test_host_reciprocal_within_1_ulp(i2):
+    """rcp_pos from a single-precision start (the device starts from the
+    hardware estimate): three Newton steps reach 1 ulp."""
+    import mpmath
+    mpmath.mp.prec = 120
+    rng = np.random.default_rng(3)
+    x = np.exp(rng.uniform(-690, 690, 4000)) * rng.choice([-1.0, 1.0], 4000)
+    x = np.concatenate([x, [1.0, 3.0, 0.1, 2.0 ** -999, 2.0 ** 999]])
+    got = _host_math(i2, 6, x)
+    e = _ulp_err(got, [1 / mpmath.mpf(v) for v in x])
+    assert e.max() <= 1.0, e.max()
 
 
 def _host_theta_psi(i2, fast, pts, tri9):

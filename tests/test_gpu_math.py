@@ -1,6 +1,6 @@
-"""The kernels' FP64 sqrt, log and atan2 on the device (hardware 1/sqrt
-estimate, LDS log table) within 1-2 ulp of 120-bit references, like their
-host builds (test_cpu_library)."""
+"""The kernels' FP64 sqrt, log, atan2 and reciprocal on the device (hardware
+1/sqrt and 1/x estimates, LDS log table) within 1-2 ulp of 120-bit
+references, like their host builds (test_cpu_library)."""
 import ctypes
 
 import numpy as np
@@ -32,8 +32,9 @@ def test_device_sqrt_log_atan2(gpu):
     y = rng.normal(size=3000) * np.exp(rng.uniform(-20, 20, 3000))
     xx = rng.normal(size=3000) * np.exp(rng.uniform(-20, 20, 3000))
     e_atan = _ulp_err(device_math(gpu, 1, xx, y), [mpmath.atan2(mpmath.mpf(a), mpmath.mpf(b)) for a, b in zip(y, xx)])
+    e_rcp = _ulp_err(device_math(gpu, 6, x), [1 / mpmath.mpf(v) for v in x])
     lib_sqrt = _ulp_err(device_math(gpu, 5, x), [mpmath.sqrt(mpmath.mpf(v)) for v in x])
     lib_log = _ulp_err(device_math(gpu, 3, x), [mpmath.log(mpmath.mpf(v)) for v in x])
     print(f"device ulp errors: sqrt {e_sqrt.max():.2f} (library {lib_sqrt.max():.2f}), log {e_log.max():.2f} "
-          f"(library {lib_log.max():.2f}), atan2 {e_atan.max():.2f}", flush=True)
-    assert e_sqrt.max() <= 1.0 and e_log.max() <= 2.0 and e_atan.max() <= 2.0
+          f"(library {lib_log.max():.2f}), atan2 {e_atan.max():.2f}, reciprocal {e_rcp.max():.2f}", flush=True)
+    assert e_sqrt.max() <= 1.0 and e_log.max() <= 2.0 and e_atan.max() <= 2.0 and e_rcp.max() <= 1.0
