@@ -53,6 +53,12 @@ BASIC_OPS_PER_POINT = 188
 TRANS_PER_POINT = {"sqrt": 6, "log": 3, "atan2": 1}
 TRANS_WEIGHTS = {"sqrt": 16, "log": 92, "atan2": 99}
 POINTS = 13
+# FP64 operations this build executes per quadrature point on the sweep's
+# common path (FMA = 2; compares, selects, conversions and min/max not
+# counted): the inner integral with shared dot products, the table-driven log
+# and atan2, the refined reciprocal and the weighted sum.  Tallied from the
+# source of theta_psi_fast (i2_device.hpp) and i2_math.hpp.
+EXECUTED_FLOPS_PER_POINT = 274
 # HBM bytes per sweep dispatch on this workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE and scaled by a 1 GiB copy's readings
 # (201 MB read + 2.094 GB written; the algorithmic figure is 1.70 GB)
@@ -257,7 +263,13 @@ def main():
                                        "profiles/valu_counters_sweep_icosphere4.md",
                      "kernel": "k_sweep_disjoint (levels 0 and 1 of all disjoint pairs)", "kernel_ms": sweep_ms,
                      "flops_per_point": flops_per_point(), "points": points,
-                     "algorithmic_bytes": traffic_bytes},
+                     "algorithmic_bytes": traffic_bytes,
+                     # the reference formula's work over this build's kernel time exceeds the
+                     # peak once the kernel executes less than that work; these two figures
+                     # are the FP64 rate of the operations actually executed
+                     "executed_flops_per_point": EXECUTED_FLOPS_PER_POINT,
+                     "executed_tflops": points * EXECUTED_FLOPS_PER_POINT / (sweep_ms * 1e-3) / 1e12,
+                     "executed_frac": points * EXECUTED_FLOPS_PER_POINT / (sweep_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
     }
     out["autotune"] = tune
     progress("adjoint")

@@ -104,13 +104,13 @@ __device__ __forceinline__ double dot4(q4 a, q4 b) { return a.x * b.x + a.y * b.
 
 struct RuleSh {
     double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
-    i2m::LogTable lt;
+    i2m::MathTable lt;
 };
 __device__ __forceinline__ void load_rule_sh(RuleSh& s, const Rule& R) {
     for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
         s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
     }
-    i2m::fill_logtab(s.lt);
+    i2m::fill_mathtab(s.lt);
 }
 
 // Vertex cotangents of a panel from those of its tangents and normal
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_disjoint(AdjArgs A) {
             Weights W = {0.0, {0.0, 0.0, 0.0}};
             if (valid) W = weights(A.seed, i, j, M.nT);
             const q4 gb = task_cotangent(W, J.n);
-            const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+            const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
             TPBar g = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             q4 F = q4zero();
             const int nsub = lev == 1 ? 4 : 1;
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_deep(AdjArgs A) {
         const Panel J = M.panels[e.j];
         const Weights W = weights(A.seed, e.i, e.j, M.nT);
         const q4 gb = task_cotangent(W, J.n);
-        const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+        const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
         double fa = I.area;
         for (int l = 0; l < e.level; ++l) fa *= 0.25;
         TPBar g = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -349,7 +349,7 @@ __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R,
     const Panel J = M.panels[j];
     const Weights W = weights(A.seed, i, j, M.nT);
     const q4 gb = task_cotangent(W, J.n);
-    const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+    const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
     int s1, s2;
     EdgeCtx ec;
     VertexCtx vc;
@@ -677,9 +677,9 @@ int i2_host_point_adjoint(int kind, const double* p3, const double* params, cons
     d3 pb;
     q4 f;
     if (kind == 0) {
-        const JGeom J = {v(0), v(3), v(6), v(9), v(12), v(15)};
+        const JGeom J = make_jgeom(v(0), v(3), v(6), v(9), v(12), v(15));
         TPBar g = {z3, z3, z3, z3, z3, z3};
-        f = theta_psi_fast_rev(p, J, &i2m::h_logtab, gb, &pb, g);
+        f = theta_psi_fast_rev(p, J, &i2m::h_mathtab, gb, &pb, g);
         put(0, g.a); put(3, g.b); put(6, g.c); put(9, g.ta); put(12, g.tb); put(15, g.tc);
     } else if (kind == 1) {
         const EdgeCtx c = {v(0), v(3), v(6), v(9), v(12), params[15]};

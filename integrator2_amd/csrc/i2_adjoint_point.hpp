@@ -18,7 +18,7 @@ I2_HD __forceinline__ void axpy(d3& y, double a, d3 x) { y.x += a * x.x; y.y += 
 // cotangents of the parameters of thetaPsi over K_j: vertices and edge tangents
 struct TPBar { d3 a, b, c, ta, tb, tc; };
 
-I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::LogTable* __restrict__ lt, q4 gb, d3* pbar,
+I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::MathTable* __restrict__ lt, q4 gb, d3* pbar,
                                             TPBar& g) {
     // ---- forward, as theta_psi_fast
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
@@ -37,7 +37,7 @@ I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::Log
     r.z = t1 * J.tc.z + t2 * J.ta.z + t3 * J.tb.z;
     const double y = dot(cross(da, db), dc);
     const double x = la * lb * lc + dot(da, db) * lc + dot(db, dc) * la + dot(dc, da) * lb;
-    r.w = 2.0 * i2m::atan2_fast(y, x);
+    r.w = 2.0 * i2m::atan2_fast(y, x, lt);
     // ---- reverse
     const d3 gP = {gb.x, gb.y, gb.z};
     const double tb1 = dot(gP, J.tc), tb2 = dot(gP, J.ta), tb3 = dot(gP, J.tb);

@@ -63,7 +63,7 @@ __device__ __forceinline__ int find_base(const int64_t* base, int n, int64_t x) 
     return lo;
 }
 
-__device__ __forceinline__ JGeom jgeom(const Panel& P) { return {P.a, P.b, P.c, P.ta, P.tb, P.tc}; }
+__device__ __forceinline__ JGeom jgeom(const Panel& P) { return make_jgeom(P.a, P.b, P.c, P.ta, P.tb, P.tc); }
 
 template <int TYPE> struct Sing;
 template <> struct Sing<I2_NOT_NEIGHBORS> {
@@ -105,14 +105,14 @@ __device__ __noinline__ q4 singular_integral(int type, const Panel& I, const Pan
 // Rule held in LDS: (L1, L2, L3, w) per point, and the log reduction table
 struct RuleLDS {
     double L1[MAX_POINTS], L2[MAX_POINTS], L3[MAX_POINTS], w[MAX_POINTS];
-    i2m::LogTable lt;
+    i2m::MathTable lt;
 };
 
 __device__ __forceinline__ void load_rule(RuleLDS& s, const Rule& R) {
     for (int k = threadIdx.x; k < MAX_POINTS; k += blockDim.x) {
         s.L1[k] = R.L1[k]; s.L2[k] = R.L2[k]; s.L3[k] = R.L3[k]; s.w[k] = R.w[k];
     }
-    i2m::fill_logtab(s.lt);
+    i2m::fill_mathtab(s.lt);
 }
 
 // Partial quadrature sum_k w_k f(r_k) of one sub-panel over points k = s, s + step, ...
@@ -278,7 +278,7 @@ __device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS&
 // Disjoint pairs, levels 0 and 1: item = (mesh, control panel i, P consecutive j),
 // processed in rounds of blockDim/LPP pairs.
 template <int OUT, int LPP>
-__global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_disjoint(BatchArgs A) {
+__global__ __launch_bounds__(256) I2_THREE_WAVES void k_sweep_disjoint(BatchArgs A) {
     __shared__ RuleLDS R;
     __shared__ PtsLDS pts;
     __shared__ long long item_sh;

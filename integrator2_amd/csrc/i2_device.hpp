@@ -18,6 +18,9 @@
 // wavefronts fit per SIMD; the quadrature loop's latency (LDS table loads,
 // transcendental chains) needs the second wavefront to hide it.
 #define I2_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2)))
+// The disjoint sweep fits three (168 registers, a few spills outside its
+// quadrature loop), which measured 7 % faster than two and than four.
+#define I2_THREE_WAVES __attribute__((amdgpu_waves_per_eu(3)))
 
 #include "i2_math.hpp"
 
@@ -135,7 +138,7 @@ I2_HD __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, 
 // (Van Oosterom and Strackee, IEEE Trans. Biomed. Eng. 30 (1983) 125-126):
 // identical in exact arithmetic, with 3 square roots, 1 reciprocal, 3 logs
 // and 1 atan2 per point instead of 13 divisions (sqrt, log and atan2 from
-// i2_math.hpp, within 2 ulp; `lt` is the log reduction table, in LDS on the
+// i2_math.hpp, within 2 ulp; `lt` holds their reduction tables, in LDS on the
 // device).  The degenerate-branch tests |1 + (d/l).tau| < eps become
 // |l + d.tau| < eps l.
 //
@@ -146,44 +149,75 @@ I2_HD __forceinline__ q4 theta_psi(d3 p, const d3& A, const d3& B, const d3& C, 
 // the same number without cancellation, so these points come out more
 // accurate than the reference formula.  The branch is rare for disjoint pairs
 // and skipped by whole wavefronts.
-struct JGeom { d3 a, b, c, ta, tb, tc; };
+// K_j for the inner integral: vertices, edge tangents (as in the Panel
+// record), edge lengths La = |C - B|, Lb = |A - C|, Lc = |B - A| and
+// N2 = (B - A) x (C - A) (twice the area times the unit normal)
+struct JGeom { d3 a, b, c, ta, tb, tc; double La, Lb, Lc; d3 N2; };
+I2_HD __forceinline__ JGeom make_jgeom(d3 a, d3 b, d3 c, d3 ta, d3 tb, d3 tc) {
+    return {a, b, c, ta, tb, tc, vlen(c - b), vlen(a - c), vlen(b - a), cross(b - a, c - a)};
+}
 I2_HD __forceinline__ double l_plus_dtau(double l, d3 d, d3 t) {
     const d3 c = cross(d, t);
     return dot(c, c) / (l - dot(d, t));
 }
-// the six l + d.tau terms of the inner integral and the reference's degenerate branch
+// Rare branch of the six l + d.tau terms: terms below 1e-4 l recomputed
+// without cancellation, then the reference's degenerate branch (a point on an
+// edge line of K_j keeps the ratio of distances, evaluatorJ3DK.cu:294-307).
+I2_HD __forceinline__ void fix_terms(const JGeom& J, d3 da, d3 db, d3 dc, double la, double lb, double lc,
+                                     double& n1, double& e1, double& n2, double& e2, double& n3, double& e3,
+                                     bool& f1, bool& f2, bool& f3) {
+    const double s = 1e-4;
+    if (n1 < s * la) n1 = l_plus_dtau(la, da, J.tc);
+    if (e1 < s * lb) e1 = l_plus_dtau(lb, db, J.tc);
+    if (n2 < s * lb) n2 = l_plus_dtau(lb, db, J.ta);
+    if (e2 < s * lc) e2 = l_plus_dtau(lc, dc, J.ta);
+    if (n3 < s * lc) n3 = l_plus_dtau(lc, dc, J.tb);
+    if (e3 < s * la) e3 = l_plus_dtau(la, da, J.tb);
+    const double h = 0.5 * EPS_PSI_THETA2;
+    f1 = fabs(e1) < h * lb; f2 = fabs(e2) < h * lc; f3 = fabs(e3) < h * la;
+    if (f1) { n1 = lb; e1 = la; }
+    if (f2) { n2 = lc; e2 = lb; }
+    if (f3) { n3 = la; e3 = lc; }
+}
+// one test per distance for the rare branch (the per-term tests are repeated
+// inside it rather than kept live through the common path)
+I2_HD __forceinline__ bool terms_need_fix(double la, double lb, double lc, double n1, double e1, double n2, double e2,
+                                          double n3, double e3) {
+    const double s = 1e-4;
+    return fmin(fmin(fmin(n1, e3) - s * la, fmin(e1, n2) - s * lb), fmin(e2, n3) - s * lc) < 0.0;
+}
+// the six l + d.tau terms of the inner integral, each from its own dot product
 I2_HD __forceinline__ void theta_psi_terms(const JGeom& J, d3 da, d3 db, d3 dc, double la, double lb, double lc,
                                            double& n1, double& e1, double& n2, double& e2, double& n3, double& e3,
                                            bool& f1, bool& f2, bool& f3) {
     n1 = la + dot(da, J.tc); e1 = lb + dot(db, J.tc);
     n2 = lb + dot(db, J.ta); e2 = lc + dot(dc, J.ta);
     n3 = lc + dot(dc, J.tb); e3 = la + dot(da, J.tb);
-    const double s = 1e-4;
     f1 = f2 = f3 = false;
-    // one test per distance (the per-term tests are repeated inside the rare
-    // branch rather than kept live through the common path)
-    const double m = fmin(fmin(fmin(n1, e3) - s * la, fmin(e1, n2) - s * lb), fmin(e2, n3) - s * lc);
-    if (__builtin_expect(m < 0.0, 0)) {
-        if (n1 < s * la) n1 = l_plus_dtau(la, da, J.tc);
-        if (e1 < s * lb) e1 = l_plus_dtau(lb, db, J.tc);
-        if (n2 < s * lb) n2 = l_plus_dtau(lb, db, J.ta);
-        if (e2 < s * lc) e2 = l_plus_dtau(lc, dc, J.ta);
-        if (n3 < s * lc) n3 = l_plus_dtau(lc, dc, J.tb);
-        if (e3 < s * la) e3 = l_plus_dtau(la, da, J.tb);
-        // a point on an edge line of K_j keeps the ratio of distances (evaluatorJ3DK.cu:294-307)
-        const double h = 0.5 * EPS_PSI_THETA2;
-        f1 = fabs(e1) < h * lb; f2 = fabs(e2) < h * lc; f3 = fabs(e3) < h * la;
-        if (f1) { n1 = lb; e1 = la; }
-        if (f2) { n2 = lc; e2 = lb; }
-        if (f3) { n3 = la; e3 = lc; }
-    }
+    if (__builtin_expect(terms_need_fix(la, lb, lc, n1, e1, n2, e2, n3, e3), 0))
+        fix_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
 }
-I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogTable* __restrict__ lt) {
+// The forward kernels reuse three of the terms' dot products in the solid
+// angle: with db = da - Lc tc (and cyclically), da.db = |da|^2 - Lc da.tc,
+// db.dc = |db|^2 - La db.ta, dc.da = |dc|^2 - Lb dc.tb; and
+// da.(db x dc) = da.((B - A) x (C - A)).  The rounding of tc, Lc and N2 makes
+// these a fixed relative 1e-16 perturbation of x and y, which moves Theta by a
+// relative 1e-16.  The terms themselves keep their own dot products: there a
+// fixed absolute error of ulp(Lc) (from db.tc = da.tc - Lc) is relative to
+// l + d.tau, which can be much smaller than Lc, and it does not average out
+// over the points as rounding noise does.
+I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::MathTable* __restrict__ lt) {
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
-    const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
-    double n1, e1, n2, e2, n3, e3;
-    bool f1, f2, f3;
-    theta_psi_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
+    const double qa = dot(da, da), qb = dot(db, db), qc = dot(dc, dc);
+    const double la = i2m::sqrt_pos(qa), lb = i2m::sqrt_pos(qb), lc = i2m::sqrt_pos(qc);
+    const double s1 = dot(da, J.tc), s2 = dot(db, J.ta), s3 = dot(dc, J.tb);
+    double n1 = la + s1, e1 = lb + dot(db, J.tc);
+    double n2 = lb + s2, e2 = lc + dot(dc, J.ta);
+    double n3 = lc + s3, e3 = la + dot(da, J.tb);
+    if (__builtin_expect(terms_need_fix(la, lb, lc, n1, e1, n2, e2, n3, e3), 0)) {
+        bool f1, f2, f3;
+        fix_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
+    }
     const double e12 = e1 * e2;
     const double inv = i2m::rcp_pos(e12 * e3);
     const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);
@@ -193,9 +227,9 @@ I2_HD __forceinline__ q4 theta_psi_fast(d3 p, const JGeom& J, const i2m::LogTabl
     r.x = t1 * J.tc.x + t2 * J.ta.x + t3 * J.tb.x;
     r.y = t1 * J.tc.y + t2 * J.ta.y + t3 * J.tb.y;
     r.z = t1 * J.tc.z + t2 * J.ta.z + t3 * J.tb.z;
-    const double y = dot(cross(da, db), dc);
-    const double x = la * lb * lc + dot(da, db) * lc + dot(db, dc) * la + dot(dc, da) * lb;
-    r.w = 2.0 * i2m::atan2_fast(y, x);
+    const double y = dot(da, J.N2);
+    const double x = la * lb * lc + fma(-J.Lc, s1, qa) * lc + fma(-J.La, s2, qb) * la + fma(-J.Lb, s3, qc) * lb;
+    r.w = 2.0 * i2m::atan2_fast(y, x, lt);
     return r;
 }
 

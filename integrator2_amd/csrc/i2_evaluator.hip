@@ -81,7 +81,7 @@ template <> struct PairCtx<I2_SIMPLE_NEIGHBORS> {
 // quadrature point lies near the extension of an edge line of K_j, which
 // moves Runge decisions; both paths therefore take decisions on accurate values.
 template <int TYPE>
-__device__ __forceinline__ q4 quad_sub(const Rule& R, const i2m::LogTable* lt, d3 a, d3 b, d3 c, double area,
+__device__ __forceinline__ q4 quad_sub(const Rule& R, const i2m::MathTable* lt, d3 a, d3 b, d3 c, double area,
                                        const JGeom& J, const PairCtx<TYPE>& ctx) {
     q4 acc = q4zero();
     for (int k = 0; k < R.n; ++k) {
@@ -101,8 +101,8 @@ template <int TYPE, bool IMPLICIT>
 __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __restrict__ list, TaskSource src,
                                                const Panel* __restrict__ panels, Rule R, int level, int lanes_log2,
                                                double4* __restrict__ out) {
-    __shared__ i2m::LogTable lt;
-    i2m::fill_logtab(lt);
+    __shared__ i2m::MathTable lt;
+    i2m::fill_mathtab(lt);
     __syncthreads();
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int G = 1 << lanes_log2;
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_level(int64_t nitems, const int32_t* __
         task_ij<IMPLICIT>(src, z, &i, &j);
         const Panel I = panels[i];
         const Panel J = panels[j];
-        const JGeom JG{J.a, J.b, J.c, J.ta, J.tb, J.tc};
+        const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
         PairCtx<TYPE> ctx;
         ctx.init(I, J);
         double area = I.area;

@@ -13,17 +13,17 @@
 //          Goldschmidt step g = s y, h = y/2, r = 1/2 - g h, then
 //          g + (s - g^2) h (one correction; within 1 ulp, not always
 //          correctly rounded).
-//   log:   x = 2^k m with m in [sqrt(1/2), sqrt(2)); j = rint(64 (m - 1)) picks
-//          c_j = 1 + j/64 from a 47-entry table of invc_j = 1/c_j (rounded) and
-//          T_j = -log(invc_j) (hi + lo); r = m invc_j - 1 by one fma, |r| < 0.0112;
-//          log1p(r) by its Taylor series to r^8 (truncation below 2^-56
-//          relative); log x = (k ln2_hi + T_hi) + (k ln2_lo + T_lo + log1p r).
-//          c_0 = 1 exactly, so there is no cancellation near x = 1.  No division.
-//   atan2: reduce to t = min(|x|,|y|)/max(|x|,|y|) in [0, 1]; with a centre
-//          c in {0, c8, 1} (c8 = the double nearest tan(pi/8)),
-//          atan t = atan c + atan u, u = (n - c d)/(d + c n) for t = n/d, so
-//          |u| <= tan(pi/16); atan u by its Taylor series to u^21/21;
-//          then the octant and quadrant identities.
+//   log:   x = 2^k m with m in [1/2, 1) (frexp); j = rint(512 m) - 256 picks
+//          c_j = 1/2 + j/512 from a 257-entry table of invc_j = 1/c_j (rounded)
+//          and T_j = -log(invc_j) (hi + lo); r = m invc_j - 1 by one fma,
+//          |r| <= 2^-9; log1p(r) by its Taylor series to r^6; log x =
+//          (k ln2_hi + T_hi) + (k ln2_lo + T_lo + log1p r).  c_256 = 1 and
+//          c_0 = 1/2 exactly, so there is no cancellation near x = 1.  No division.
+//   atan2: reduce to t = min(|x|,|y|)/max(|x|,|y|) in [0, 1]; the nearest
+//          centre c = k/32 (from a reciprocal estimate) and a 33-entry table of
+//          atan(k/32) (hi + lo) give atan t = atan c + atan u with
+//          u = (n - c d)/(d + c n) for t = n/d, |u| <= 2^-6; atan u by its
+//          Taylor series to u^9/9; then the octant and quadrant identities.
 //   1/x and a/b (rcp_pos, div_pos): hardware reciprocal estimate refined by
 //          Newton steps with fma residuals; used where the operand is a
 //          product of distances, far inside [2^-1000, 2^1000].
@@ -42,15 +42,8 @@ namespace i2m {
 
 constexpr double LN2_HI = 0.69314718036912382;         // ln 2 to 32 significant bits
 constexpr double LN2_LO = 1.9082149292705877e-10;      // ln 2 - LN2_HI
-constexpr double SQRT_HALF = 0.70710678118654757;
 constexpr double PI_HI = 3.1415926535897931, PI_LO = 1.2246467991473532e-16;
 constexpr double PIO2_HI = 1.5707963267948966, PIO2_LO = 6.123233995736766e-17;
-constexpr double PIO4_HI = 0.78539816339744828, PIO4_LO = 3.061616997868383e-17;
-constexpr double C8 = 0.41421356237309503;             // double nearest tan(pi/8)
-constexpr double ATAN_C8_HI = 0.39269908169872414;     // atan(C8), split
-constexpr double ATAN_C8_LO = 3.060132146563891e-18;
-constexpr double T16 = 0.19891236737965801;            // tan(pi/16)
-constexpr double T316 = 0.66817863791929888;           // tan(3 pi/16)
 
 // library calls for arguments outside the fast paths, kept out of line
 static I2_HD __noinline__ double log_slow(double x) { return ::log(x); }
@@ -119,53 +112,64 @@ I2_HD __forceinline__ double fma_k(double a, double b, double c) {
 #endif
 }
 
-// log reduction table (gen_log_table.py): entry j + LOGTAB_J0 for c_j = 1 + j/64,
-// held as three arrays so that an entry's address is a shift of j
+// Reduction tables (gen_math_tables.py), held as arrays so that an entry's
+// address is a shift of its index: log entries j = 0..256 for the centres
+// c_j = 1/2 + j/512, atan entries k = 0..32 for atan(k/32).
 struct LogEntry { double invc, thi, tlo; };
-struct LogTable { double invc[47], thi[47], tlo[47]; };
-constexpr int LOGTAB_J0 = 19;
-constexpr int LOGTAB_N = 47;
+struct AtanEntry { double hi, lo; };
+constexpr int LOGTAB_N = 257;
+constexpr int ATANTAB_N = 33;
+struct MathTable {
+    double invc[LOGTAB_N], thi[LOGTAB_N], tlo[LOGTAB_N];
+    AtanEntry atan[ATANTAB_N];
+};
 constexpr LogEntry h_logent[LOGTAB_N] = {
 #include "i2_logtab.inc"
 };
-constexpr LogTable make_logtab(const LogEntry (&e)[LOGTAB_N]) {
-    LogTable t{};
-    for (int k = 0; k < LOGTAB_N; ++k) { t.invc[k] = e[k].invc; t.thi[k] = e[k].thi; t.tlo[k] = e[k].tlo; }
+constexpr AtanEntry h_atanent[ATANTAB_N] = {
+#include "i2_atantab.inc"
+};
+constexpr MathTable make_mathtab() {
+    MathTable t{};
+    for (int k = 0; k < LOGTAB_N; ++k) { t.invc[k] = h_logent[k].invc; t.thi[k] = h_logent[k].thi; t.tlo[k] = h_logent[k].tlo; }
+    for (int k = 0; k < ATANTAB_N; ++k) t.atan[k] = h_atanent[k];
     return t;
 }
-static constexpr LogTable h_logtab = make_logtab(h_logent);
-static __constant__ LogEntry c_logtab[LOGTAB_N] = {
+static constexpr MathTable h_mathtab = make_mathtab();
+static __constant__ LogEntry c_logent[LOGTAB_N] = {
 #include "i2_logtab.inc"
 };
-// copies the table into `s` (LDS) with the threads of the block; the caller synchronises
-__device__ __forceinline__ void fill_logtab(LogTable& s) {
+static __constant__ AtanEntry c_atanent[ATANTAB_N] = {
+#include "i2_atantab.inc"
+};
+// copies the tables into `s` (LDS) with the threads of the block; the caller synchronises
+__device__ __forceinline__ void fill_mathtab(MathTable& s) {
     for (int k = threadIdx.x; k < LOGTAB_N; k += blockDim.x) {
-        s.invc[k] = c_logtab[k].invc; s.thi[k] = c_logtab[k].thi; s.tlo[k] = c_logtab[k].tlo;
+        s.invc[k] = c_logent[k].invc; s.thi[k] = c_logent[k].thi; s.tlo[k] = c_logent[k].tlo;
     }
+    for (int k = threadIdx.x; k < ATANTAB_N; k += blockDim.x) s.atan[k] = c_atanent[k];
 }
 
 // log of a positive normal number with the reduction table `tab` (LDS on the device)
-I2_HD __forceinline__ double log_tab(double x, const LogTable* __restrict__ tab) {
+I2_HD __forceinline__ double log_tab(double x, const MathTable* __restrict__ tab) {
     if (!(x >= 2.2250738585072014e-308 && x <= 1.7976931348623157e308)) return log_slow(x);
     int k;
-    double m = frexp(x, &k);             // x = m 2^k, m in [0.5, 1)
-    if (m < SQRT_HALF) { m += m; --k; }  // m in [sqrt(1/2), sqrt(2))
-    // nearest centre (ties upward): 64 m + 1/2 in [45.7, 91.1), so j in [-19, 27]
-    const int j = (int)fma(m, 64.0, 0.5) - 64 + LOGTAB_J0;
-    const double r = fma(m, tab->invc[j], -1.0);
-    // log1p(r) = r - r^2/2 + ... - r^8/8
-    double q = fma_k(r, -1.0 / 8.0, 1.0 / 7.0);
-    q = fma_k(q, r, -1.0 / 6.0);
-    q = fma_k(q, r, 1.0 / 5.0);
-    q = fma(q, r, -1.0 / 4.0);
+    const double m = frexp(x, &k);                 // x = m 2^k, m in [1/2, 1)
+    const int j = (int)fma(m, 512.0, 0.5) - 256;   // nearest centre (ties upward), j in [0, 256]
+    const double r = fma(m, tab->invc[j], -1.0);   // |r| <= 2^-9
+    // log1p(r) = r - r^2/2 + r^3/3 - r^4/4 + r^5/5 - r^6/6 (truncation below 2^-56 relative)
+    double q = fma_k(r, -1.0 / 6.0, 1.0 / 5.0);
+    q = fma(q, r, -0.25);
     q = fma_k(q, r, 1.0 / 3.0);
     q = fma(q, r, -0.5);
     const double p = fma(r * r, q, r);
+    // k ln2_hi + T_hi is exact: k ln2_hi has 53 bits, and for k = 1 and
+    // c_j < sqrt(1/2) the two terms cancel by Sterbenz' lemma
     const double dk = (double)k;
     return fma(dk, LN2_HI, tab->thi[j]) + (fma(dk, LN2_LO, tab->tlo[j]) + p);
 }
 
-I2_HD __forceinline__ double atan2_fast(double y, double x) {
+I2_HD __forceinline__ double atan2_fast(double y, double x, const MathTable* __restrict__ tab) {
     const double ax = fabs(x), ay = fabs(y);
     // zero, tiny, huge, infinite and NaN arguments take the library path
     const double sum = ax + ay;
@@ -173,27 +177,21 @@ I2_HD __forceinline__ double atan2_fast(double y, double x) {
     const bool swap = ay > ax;
     const double n = fmin(ax, ay);
     const double d = fmax(ax, ay);                   // t = n/d in [0, 1], d >= 2^-1000
-    const bool mid = n > T16 * d;
-    const bool high = n > T316 * d;
-    const double c = high ? 1.0 : (mid ? C8 : 0.0);
-    const double bhi = high ? PIO4_HI : (mid ? ATAN_C8_HI : 0.0);
-    const double blo = high ? PIO4_LO : (mid ? ATAN_C8_LO : 0.0);
-    const double u = div_pos(fma(-c, d, n), fma(c, n, d));   // |u| <= tan(pi/16)
+    // centre c = k/32 nearest to t, from the reciprocal estimate (relative error
+    // far below 2^-10): k in [0, 32] and |t - c| <= 2^-6 (1 + 2^-10)
+    const int k = (int)fma(n * rcp_estimate(d), 32.0, 0.5);
+    const double c = (double)k * (1.0 / 32.0);
+    const double u = div_pos(fma(-c, d, n), fma(c, n, d));   // atan t = atan c + atan u
     const double w = u * u;
-    // -1/3 + w/5 - ... + w^10/21
-    double p = fma_k(w, 1.0 / 21.0, -1.0 / 19.0);
-    p = fma_k(p, w, 1.0 / 17.0);
-    p = fma_k(p, w, -1.0 / 15.0);
-    p = fma_k(p, w, 1.0 / 13.0);
-    p = fma_k(p, w, -1.0 / 11.0);
-    p = fma_k(p, w, 1.0 / 9.0);
-    p = fma_k(p, w, -1.0 / 7.0);
+    // atan u = u - u^3/3 + u^5/5 - u^7/7 + u^9/9 (truncation below 2^-60 relative)
+    double p = fma_k(w, 1.0 / 9.0, -1.0 / 7.0);
     p = fma_k(p, w, 1.0 / 5.0);
     p = fma_k(p, w, -1.0 / 3.0);
-    const double au = fma(u * w, p, u);               // atan u
-    double r = bhi + (au + blo);                      // atan t, t in [0, 1]
-    if (swap) r = PIO2_HI - r + PIO2_LO;              // atan(1/t) = pi/2 - atan t
-    if (x < 0.0) r = PI_HI - r + PI_LO;               // second and third quadrants
+    const double au = fma(u * w, p, u);
+    const AtanEntry b = tab->atan[k];
+    double r = b.hi + (au + b.lo);                   // atan t, t in [0, 1]
+    if (swap) r = PIO2_HI - r + PIO2_LO;             // atan(1/t) = pi/2 - atan t
+    if (x < 0.0) r = PI_HI - r + PI_LO;              // second and third quadrants
     return (y < 0.0 || (y == 0.0 && signbit(y))) ? -r : r;
 }
 

@@ -141,9 +141,9 @@ const char* i2_last_error(void) { return g_last_error.c_str(); }
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out) {
     if (!x || !out || n < 0 || (fn == 1 && !y)) { set_error("null argument"); return I2_ERR_INVALID; }
     for (int64_t q = 0; q < n; ++q) {
-        if (fn == 0) out[q] = i2m::log_tab(x[q], &i2m::h_logtab);
+        if (fn == 0) out[q] = i2m::log_tab(x[q], &i2m::h_mathtab);
         else if (fn == 2) out[q] = i2m::sqrt_pos(x[q]);
-        else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
+        else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q], &i2m::h_mathtab);
         else if (fn == 6) out[q] = i2m::rcp_pos(x[q]);
         else { set_error("unknown function %d", fn); return I2_ERR_INVALID; }
     }
@@ -154,10 +154,10 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
     if (!pts || !tri9 || !out4 || n < 0) { set_error("null argument"); return I2_ERR_INVALID; }
     using namespace i2d;
     const d3 A = {tri9[0], tri9[1], tri9[2]}, B = {tri9[3], tri9[4], tri9[5]}, C = {tri9[6], tri9[7], tri9[8]};
-    const JGeom J = {A, B, C, unit(C - B), unit(A - C), unit(B - A)};
+    const JGeom J = make_jgeom(A, B, C, unit(C - B), unit(A - C), unit(B - A));
     for (int64_t q = 0; q < n; ++q) {
         const d3 p = {pts[3 * q], pts[3 * q + 1], pts[3 * q + 2]};
-        const q4 r = fast ? theta_psi_fast(p, J, &i2m::h_logtab) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
+        const q4 r = fast ? theta_psi_fast(p, J, &i2m::h_mathtab) : theta_psi(p, J.a, J.b, J.c, J.ta, J.tb, J.tc);
         out4[4 * q] = r.x; out4[4 * q + 1] = r.y; out4[4 * q + 2] = r.z; out4[4 * q + 3] = r.w;
     }
     return I2_OK;
@@ -165,13 +165,13 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
 
 // The kernels' math functions on the device, for checking (fn as in i2_host_math).
 __global__ void k_math(int fn, int64_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ out) {
-    __shared__ i2m::LogTable lt;
-    i2m::fill_logtab(lt);
+    __shared__ i2m::MathTable lt;
+    i2m::fill_mathtab(lt);
     __syncthreads();
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     if (fn == 0) out[q] = i2m::log_tab(x[q], &lt);
-    else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q]);
+    else if (fn == 1) out[q] = i2m::atan2_fast(y[q], x[q], &lt);
     else if (fn == 2) out[q] = i2m::sqrt_pos(x[q]);
     else if (fn == 3) out[q] = log(x[q]);
     else if (fn == 4) out[q] = atan2(y[q], x[q]);
@@ -219,7 +219,7 @@ int i2_host_task_value(const double* I9, const double* J9, const int32_t* vid6, 
     };
     const Panel I = panel(I9, vid6), J = panel(J9, vid6 + 3);
     const int shared = shared_count(I, J);
-    const JGeom JG = {J.a, J.b, J.c, J.ta, J.tb, J.tc};
+    const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
     int s1 = 0, s2 = 0;
     EdgeCtx ec{};
     VertexCtx vc{};
@@ -249,7 +249,7 @@ int i2_host_task_value(const double* I9, const double* J9, const int32_t* vid6, 
             d3 p = l1 * a;
             p = p + l2 * b;
             p = p + l3 * cc;
-            q4 f = theta_psi_fast(p, JG, &i2m::h_logtab);
+            q4 f = theta_psi_fast(p, JG, &i2m::h_mathtab);
             if (shared == 2) f = f - sing_edge(p, ec);
             if (shared == 1) f = f - sing_vertex(p, vc);
             acc = acc + w[k] * f;

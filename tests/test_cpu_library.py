@@ -58,7 +58,11 @@ def test_host_log_within_2_ulp(i2):
     import mpmath
     mpmath.mp.prec = 120
     rng = np.random.default_rng(0)
-    x = np.concatenate([np.exp(rng.uniform(-700, 700, 3000)), 1.0 + rng.uniform(-1e-3, 1e-3, 1000),
+    # table centre ties (m = 1/2 + (j + 1/2)/512) and the neighbourhood of 1, where
+    # k ln2 must cancel the first table entry exactly
+    ties = (0.5 + (np.arange(256) + 0.5) / 512) * 2.0 ** rng.integers(-5, 5, size=256)
+    near1 = 1.0 + np.concatenate([rng.uniform(-1e-9, 1e-9, 500), [1e-15, -1e-15, 2.0 ** -52, -2.0 ** -53, 1e-12]])
+    x = np.concatenate([np.exp(rng.uniform(-700, 700, 3000)), 1.0 + rng.uniform(-1e-3, 1e-3, 1000), ties, near1,
                         [1.0, 2.0, 0.5, np.sqrt(0.5), np.sqrt(2.0), 2.2250738585072014e-308, 1.7976931348623157e308]])
     got = _host_math(i2, 0, x)
     e = _ulp_err(got, [mpmath.log(mpmath.mpf(v)) for v in x])
@@ -85,10 +89,12 @@ def test_host_atan2_within_2_ulp(i2):
     rng = np.random.default_rng(1)
     y = rng.normal(size=4000) * np.exp(rng.uniform(-30, 30, 4000))
     x = rng.normal(size=4000) * np.exp(rng.uniform(-30, 30, 4000))
-    # octant and centre boundaries
-    t = np.array([0.0, 0.19891236737965801, 0.41421356237309503, 0.66817863791929888, 1.0])
+    # octant boundaries and the midpoints between the centres k/32, with their neighbours
+    mids = (np.arange(32) + 0.5) / 32
+    t = np.concatenate([[0.0, 1.0], mids, np.nextafter(mids, 0.0), np.nextafter(mids, 1.0)])
+    nt = len(t)
     y = np.concatenate([y, t, -t, t, 1.0 / np.maximum(t, 1e-300)])
-    x = np.concatenate([x, np.ones(5), np.ones(5), -np.ones(5), np.ones(5)])
+    x = np.concatenate([x, np.ones(nt), np.ones(nt), -np.ones(nt), np.ones(nt)])
     got = _host_math(i2, 1, x, y)
     e = _ulp_err(got, [mpmath.atan2(mpmath.mpf(a), mpmath.mpf(b)) for a, b in zip(y, x)])
     assert e.max() <= 2.0, e.max()
