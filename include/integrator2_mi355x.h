@@ -59,8 +59,9 @@ int i2_host_theta_psi(int fast, const double* pts, int64_t n, const double* tri9
  * single-precision estimate).  For checking. */
 int i2_host_math(int fn, const double* x, const double* y, int64_t n, double* out);
 /* The same functions evaluated on the device (fn 0-2 and 6 as above; 3, 4,
- * 5, 7: the HIP library's log, atan2, sqrt and 1/x, for comparison).  For
- * checking. */
+ * 5, 7: the HIP library's log, atan2, sqrt and 1/x, for comparison; 8, 9:
+ * the hardware estimates of 1/x and 1/sqrt(x) that rcp_pos, div_pos and
+ * sqrt_pos refine).  For checking. */
 int i2_device_math(int device, int fn, const double* x, const double* y, int64_t n, double* out);
 /* Host-side evaluation of one ordered task's integrals (Psi, Theta) at
  * refinement level `level` with the kernels' device code compiled for the

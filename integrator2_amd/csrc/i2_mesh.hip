@@ -177,7 +177,9 @@ __global__ void k_math(int fn, int64_t n, const double* __restrict__ x, const do
     else if (fn == 4) out[q] = atan2(y[q], x[q]);
     else if (fn == 5) out[q] = sqrt(x[q]);
     else if (fn == 6) out[q] = i2m::rcp_pos(x[q]);
-    else out[q] = 1.0 / x[q];
+    else if (fn == 7) out[q] = 1.0 / x[q];
+    else if (fn == 8) out[q] = i2m::rcp_estimate(x[q]);
+    else out[q] = i2m::rsq_estimate(x[q]);
 }
 
 // Host evaluation of one ordered task's integrals with the kernels' device
@@ -187,7 +189,7 @@ __global__ void k_math(int fn, int64_t n, const double* __restrict__ x, const do
 // singular part.  tri9 are the two panels' vertices, vid6 their vertex ids
 // (they decide the neighbour type and the rotations).  For checking.
 int i2_device_math(int device, int fn, const double* x, const double* y, int64_t n, double* out) {
-    if (!x || !out || n <= 0 || fn < 0 || fn > 7 || ((fn == 1 || fn == 4) && !y)) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (!x || !out || n <= 0 || fn < 0 || fn > 9 || ((fn == 1 || fn == 4) && !y)) { set_error("invalid argument"); return I2_ERR_INVALID; }
     DeviceGuard g(device);
     if (!g.ok) return I2_ERR_HIP;
     DBuf<double> dx, dy, dout;

@@ -38,3 +38,17 @@ def test_device_sqrt_log_atan2(gpu):
     print(f"device ulp errors: sqrt {e_sqrt.max():.2f} (library {lib_sqrt.max():.2f}), log {e_log.max():.2f} "
           f"(library {lib_log.max():.2f}), atan2 {e_atan.max():.2f}, reciprocal {e_rcp.max():.2f}", flush=True)
     assert e_sqrt.max() <= 1.0 and e_log.max() <= 2.0 and e_atan.max() <= 2.0 and e_rcp.max() <= 1.0
+
+
+def test_hardware_estimates(gpu):
+    """The refinement steps of rcp_pos (two Newton steps), div_pos (one Newton
+    step and a residual correction) and sqrt_pos (one Goldschmidt step and a
+    correction) each square the relative error of their start; they reach
+    full double precision from estimates better than 2^-22."""
+    rng = np.random.default_rng(9)
+    x = np.exp(rng.uniform(-600, 600, 20000))
+    e_rcp = np.abs(device_math(gpu, 8, x) * x - 1.0).max()
+    e_rsq = np.abs(device_math(gpu, 9, x) ** 2 * x - 1.0).max() / 2
+    print(f"hardware estimates: 1/x within {e_rcp:.2e} (2^{np.log2(e_rcp):.1f}), "
+          f"1/sqrt(x) within {e_rsq:.2e} (2^{np.log2(e_rsq):.1f}) relative", flush=True)
+    assert e_rcp < 2.0 ** -22 and e_rsq < 2.0 ** -22
