@@ -24,7 +24,8 @@
 //              min(4^L, 64) lanes per task, one sub-panel per lane,
 //              fixed-order butterfly reduction.
 //   k_levels / k_compute_K   K per (mesh, type) from queue membership.
-//   k_finalize_pairs / k_finalize_rows   parity selection, J, outputs.
+//   k_finalize_disjoint (per row of the disjoint pair list), k_finalize_singular,
+//   k_finalize_rows   parity selection, J, outputs.
 #include "i2_batch.hpp"
 #include "i2_adjoint.hpp"
 
@@ -472,37 +473,59 @@ __global__ void k_compute_K(BatchArgs A) {
 }
 
 // Parity selection and J for per-pair outputs, grid-stride over all tasks.
-__global__ __launch_bounds__(256) void k_finalize_pairs(BatchArgs A) {
-    const int64_t total = A.pair_base[3 * A.nmeshes];
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
-        const int b = find_base(A.pair_base, 3 * A.nmeshes, g);
-        const int m = b / 3, type = b % 3;
+// The reported value of ordered task z of a type (I_c, or I_(c-1) where the
+// buffer swap selects it) and its J.
+__device__ __forceinline__ void finalize_task(const BatchArgs& A, const MeshDesc& M, int m, int type, int64_t z, int i, int j) {
+    const int K = A.K[3 * m + type];
+    const int c = M.conv[type][z];
+    const bool swap = A.quirk && c >= 1 && ((K - c) & 1);
+    const double4 v4 = swap ? M.prev[type][z] : M.fin[type][z];
+    const q4 v = {v4.x, v4.y, v4.z, v4.w};
+    const d3 nJ = M.panels[j].n;
+    const d3 r = (type == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(v, nJ, M.panels[i].area) : finalize_plain(v, nJ);
+    if (swap) M.fin[type][z] = v4;   // the reported value replaces I_c only where the swap selects I_(c-1)
+    double* o = M.res[type] + 3 * z;
+    o[0] = r.x; o[1] = r.y; o[2] = r.z;
+}
+
+// Disjoint tasks, one row i of the lexicographic pair list per workgroup
+// pass: the lanes walk its pairs (i, j), j > i, and finalize (i, j) and its
+// reversed task (j, i), with j from i's sorted neighbour list (no search).
+__global__ __launch_bounds__(256) void k_finalize_disjoint(BatchArgs A) {
+    for (int64_t r = blockIdx.x; r < A.total_rows; r += gridDim.x) {
+        const int m = find_base(A.row_base, A.nmeshes, r);
         const MeshDesc& M = A.meshes[m];
-        const int64_t z = g - A.pair_base[b];
-        const int64_t n = M.npairs[type];
-        const int64_t k = z < n ? z : z - n;
-        int i, j;
-        if (type == I2_NOT_NEIGHBORS) {
-            const int ii = find_base(M.row_start, M.nT, k);
-            int64_t jj = ii + 1 + (k - M.row_start[ii]);
-            for (int64_t q = M.nbr_start[ii]; q < M.nbr_start[ii + 1]; ++q) {
+        const int ii = (int)(r - A.row_base[m]);
+        const int64_t k0 = M.row_start[ii], cnt = M.row_start[ii + 1] - k0;
+        const int64_t n = M.npairs[I2_NOT_NEIGHBORS];
+        const int64_t q0 = M.nbr_start[ii], q1 = M.nbr_start[ii + 1];
+        for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+            int64_t jj = ii + 1 + t;
+            for (int64_t q = q0; q < q1; ++q) {
                 if (M.nbr[q] <= jj) ++jj; else break;
             }
-            i = ii; j = (int)jj;
-        } else {
-            i = M.lists[type][3 * k]; j = M.lists[type][3 * k + 1];
+            finalize_task(A, M, m, I2_NOT_NEIGHBORS, k0 + t, ii, (int)jj);
+            finalize_task(A, M, m, I2_NOT_NEIGHBORS, n + k0 + t, (int)jj, ii);
         }
+    }
+}
+
+// Vertex- and edge-sharing tasks, enumerated as the singular sweep's items.
+__global__ __launch_bounds__(256) void k_finalize_singular(BatchArgs A) {
+    const int64_t total = A.singular_items * 256;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t item = g >> 8;
+        const int b = find_base(A.sing_base, 2 * A.nmeshes, item);
+        const int type = b / A.nmeshes == 0 ? I2_SIMPLE_NEIGHBORS : I2_ATTACHED_NEIGHBORS;
+        const int m = b % A.nmeshes;
+        const MeshDesc& M = A.meshes[m];
+        const int64_t n = M.npairs[type];
+        const int64_t z = (item - A.sing_base[b]) * 256 + (g & 255);
+        if (z >= 2 * n) continue;
+        const int64_t k = z < n ? z : z - n;
+        int i = M.lists[type][3 * k], j = M.lists[type][3 * k + 1];
         if (z >= n) { const int t = i; i = j; j = t; }
-        const int K = A.K[3 * m + type];
-        const int c = M.conv[type][z];
-        const double4 v4 = (A.quirk && c >= 1 && ((K - c) & 1)) ? M.prev[type][z] : M.fin[type][z];
-        const q4 v = {v4.x, v4.y, v4.z, v4.w};
-        const d3 nJ = M.panels[j].n;
-        const double areaI = M.panels[i].area;
-        const d3 r = (type == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(v, nJ, areaI) : finalize_plain(v, nJ);
-        M.fin[type][z] = v4;
-        double* o = M.res[type] + 3 * z;
-        o[0] = r.x; o[1] = r.y; o[2] = r.z;
+        finalize_task(A, M, m, type, z, i, j);
     }
 }
 
@@ -624,8 +647,12 @@ int batch_enqueue(Engine& E, hipStream_t st) {
     hipLaunchKernelGGL(k_compute_K, dim3((3 * A.nmeshes + 255) / 256), dim3(256), 0, st, A);
     I2_TRY(check_launch("k_compute_K"));
     if (E.out == OUT_PER_PAIR) {
-        hipLaunchKernelGGL(k_finalize_pairs, dim3(E.fin_grid), dim3(256), 0, st, A);
-        I2_TRY(check_launch("k_finalize_pairs"));
+        hipLaunchKernelGGL(k_finalize_disjoint, dim3(E.fin_grid), dim3(256), 0, st, A);
+        I2_TRY(check_launch("k_finalize_disjoint"));
+        if (A.singular_items > 0) {
+            hipLaunchKernelGGL(k_finalize_singular, dim3(E.deep_grid), dim3(256), 0, st, A);
+            I2_TRY(check_launch("k_finalize_singular"));
+        }
     } else {
         for (int m = 0; m < A.nmeshes; ++m) {
             const int nT = (int)E.nT[m];
@@ -683,6 +710,9 @@ int engine_setup(Engine& E, const i2_integrator& it) {
     }
     E.h_pair_base[3 * M] = pb;
     E.total_tasks = pb;
+    E.h_row_base.assign(M + 1, 0);
+    for (int m = 0; m < M; ++m) E.h_row_base[m + 1] = E.h_row_base[m] + E.h_meshes[m].nT;
+    A.total_rows = E.h_row_base[M];
     // queue segments: levels 2..last, capacity per type (grown after an overflow)
     for (int t = 0; t < 3; ++t)
         if (A.qcap[t] <= 0) A.qcap[t] = std::max<int64_t>(1, std::min<int64_t>(E.ordered[t], std::max<int64_t>(65536, E.ordered[t] / 8)));
@@ -696,6 +726,7 @@ int engine_setup(Engine& E, const i2_integrator& it) {
     I2_TRY(E.d_item_base.ensure(M + 1));
     I2_TRY(E.d_sing_base.ensure(2 * M + 1));
     I2_TRY(E.d_pair_base.ensure(3 * M + 1));
+    I2_TRY(E.d_row_base.ensure(M + 1));
     I2_TRY(E.d_counters.ensure(2 + MAX_LEVELS * 3));
     I2_TRY(E.d_levelmask.ensure(MAX_LEVELS * 3 * MASK_WORDS));
     I2_TRY(E.d_K.ensure(3 * M));
@@ -704,10 +735,12 @@ int engine_setup(Engine& E, const i2_integrator& it) {
     I2_HIP(hipMemcpy(E.d_item_base.p, E.h_item_base.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice));
     I2_HIP(hipMemcpy(E.d_sing_base.p, E.h_sing_base.data(), sizeof(int64_t) * (2 * M + 1), hipMemcpyHostToDevice));
     I2_HIP(hipMemcpy(E.d_pair_base.p, E.h_pair_base.data(), sizeof(int64_t) * (3 * M + 1), hipMemcpyHostToDevice));
+    I2_HIP(hipMemcpy(E.d_row_base.p, E.h_row_base.data(), sizeof(int64_t) * (M + 1), hipMemcpyHostToDevice));
     A.meshes = E.d_meshes.p;
     A.item_base = E.d_item_base.p;
     A.sing_base = E.d_sing_base.p;
     A.pair_base = E.d_pair_base.p;
+    A.row_base = E.d_row_base.p;
     A.counters = E.d_counters.p;
     A.queue = E.d_queue.p;
     A.qoff = E.d_qoff.p;

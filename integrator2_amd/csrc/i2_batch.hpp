@@ -55,8 +55,11 @@ struct BatchArgs {
     const int64_t* sing_base;
     int64_t disjoint_items;
     int64_t singular_items;
-    // per-pair finalize ranges: pair_base[3 * m + type]
+    // per-pair finalize ranges: pair_base[3 * m + type]; disjoint rows of all
+    // meshes: row_base[m] (nmeshes + 1)
     const int64_t* pair_base;
+    const int64_t* row_base;
+    int64_t total_rows;
     // counters: [0] disjoint item counter, [1 + level*3 + type] queue counts,
     // [1 + MAX_LEVELS*3] singular item counter
     unsigned long long* counters;
@@ -77,11 +80,11 @@ struct Engine {
     int sweep_grid = 0, deep_grid = 0, fin_grid = 0;
     BatchArgs args{};
     std::vector<MeshDesc> h_meshes;
-    std::vector<int64_t> h_item_base, h_sing_base, h_pair_base, h_qoff;
+    std::vector<int64_t> h_item_base, h_sing_base, h_pair_base, h_row_base, h_qoff;
     std::vector<int64_t> nT;
     int64_t ordered[3] = {0, 0, 0};   // ordered tasks per type over the batch
     DBuf<MeshDesc> d_meshes;
-    DBuf<int64_t> d_item_base, d_sing_base, d_pair_base, d_qoff;
+    DBuf<int64_t> d_item_base, d_sing_base, d_pair_base, d_row_base, d_qoff;
     DBuf<unsigned long long> d_counters;
     DBuf<QEntry> d_queue;
     DBuf<unsigned long long> d_levelmask;
