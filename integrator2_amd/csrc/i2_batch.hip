@@ -69,7 +69,7 @@ __device__ __forceinline__ JGeom jgeom(const Panel& P) { return make_jgeom(P.a, 
 template <int TYPE> struct Sing;
 template <> struct Sing<I2_NOT_NEIGHBORS> {
     __device__ __forceinline__ void init(const Panel&, const Panel&) {}
-    __device__ __forceinline__ q4 at(d3) const { return q4zero(); }
+    __device__ __forceinline__ q4 at(d3, const i2m::MathTable*) const { return q4zero(); }
 };
 template <> struct Sing<I2_ATTACHED_NEIGHBORS> {
     EdgeCtx c;
@@ -78,7 +78,7 @@ template <> struct Sing<I2_ATTACHED_NEIGHBORS> {
         shifts_edge(I, J, &s1, &s2);
         c = edge_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3));
     }
-    __device__ __forceinline__ q4 at(d3 p) const { return sing_edge(p, c); }
+    __device__ __forceinline__ q4 at(d3 p, const i2m::MathTable* lt) const { return sing_edge_fast(p, c, lt); }
 };
 template <> struct Sing<I2_SIMPLE_NEIGHBORS> {
     VertexCtx c;
@@ -87,7 +87,7 @@ template <> struct Sing<I2_SIMPLE_NEIGHBORS> {
         shifts_vertex(I, J, &s1, &s2);
         c = vertex_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
     }
-    __device__ __forceinline__ q4 at(d3 p) const { return sing_vertex(p, c); }
+    __device__ __forceinline__ q4 at(d3 p, const i2m::MathTable* lt) const { return sing_vertex_fast(p, c, lt); }
 };
 
 // closed-form singular integral of a task (evaluatorJ3DK.cu:603-780)
@@ -126,7 +126,7 @@ __device__ __forceinline__ q4 quad_part(const RuleLDS& R, int npts, int s, int s
         p = p + R.L2[k] * b;
         p = p + R.L3[k] * c;
         q4 f = theta_psi_fast(p, J, &R.lt);
-        if (TYPE != I2_NOT_NEIGHBORS) f = f - sg.at(p);
+        if (TYPE != I2_NOT_NEIGHBORS) f = f - sg.at(p, &R.lt);
         acc = acc + R.w[k] * f;
     }
     return acc;

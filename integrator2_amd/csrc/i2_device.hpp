@@ -266,6 +266,42 @@ I2_HD __forceinline__ q4 sing_edge(d3 p, const EdgeCtx& c) {
     return r;
 }
 
+// l - t.v and l + t.v for a unit tangent t and l = |v|, recomputed as
+// |v x t|^2 / (l -+ t.v) where they fall below 1e-4 l (cancellation near the
+// line through the vertex along t).
+I2_HD __forceinline__ double l_minus_tv(double l, d3 v, d3 t, double tv) {
+    const d3 c = cross(v, t);
+    return dot(c, c) / (l + tv);
+}
+
+// The singular parts below without unit vectors, as theta_psi_fast: with
+// v = p - vertex and l = |v|, the reference's l (1 -+ t.(v/l)) is l -+ t.v
+// (|t| = 1), recomputed without cancellation when small, and the arguments of
+// each atan2 are both scaled by l > 0.  Logs and atan2 from i2_math.hpp.
+I2_HD __forceinline__ q4 sing_edge_fast(d3 p, const EdgeCtx& c, const i2m::MathTable* __restrict__ lt) {
+    const d3 va = p - c.JB, vb = p - c.JC;
+    const double lva = i2m::sqrt_pos(dot(va, va)), lvb = i2m::sqrt_pos(dot(vb, vb));
+    const double ca = dot(c.tauc, va), cb = dot(c.tauc, vb), ba = dot(c.taub, va), ab = dot(c.taua, vb);
+    double N = lvb - cb, D = lva - ca, P2 = lva + ba, P3 = lvb - ab;
+    const double s = 1e-4;
+    if (__builtin_expect(fmin(fmin(N, P3) - s * lvb, fmin(D, P2) - s * lva) < 0.0, 0)) {
+        if (N < s * lvb) N = l_minus_tv(lvb, vb, c.tauc, cb);
+        if (D < s * lva) D = l_minus_tv(lva, va, c.tauc, ca);
+        if (P2 < s * lva) P2 = l_minus_tv(lva, va, c.taub, -ba);
+        if (P3 < s * lvb) P3 = l_minus_tv(lvb, vb, c.taua, ab);
+    }
+    const double l1 = i2m::log_tab(N * i2m::rcp_pos(D), lt);
+    const double l2 = i2m::log_tab(P2 * c.ilvc, lt);
+    const double l3 = i2m::log_tab(P3 * c.ilvc, lt);
+    q4 r;
+    r.x = l1 * c.tauc.x - l2 * c.taub.x - l3 * c.taua.x;
+    r.y = l1 * c.tauc.y - l2 * c.taub.y - l3 * c.taua.y;
+    r.z = l1 * c.tauc.z - l2 * c.taub.z - l3 * c.taua.z;
+    r.w = 2.0 * (i2m::atan2_fast(dot(cross(va, c.taub), c.tauc), dot(c.taub - c.tauc, lva * c.taub + va), lt) -
+                 i2m::atan2_fast(dot(cross(vb, c.taua), c.tauc), dot(c.taua - c.tauc, lvb * c.taua - vb), lt));
+    return r;
+}
+
 // Vertex-sharing pair: JA is the shared vertex; e and the two delta angles
 // follow evaluatorJ3DK.cu:376-399 (two sign flips of e).
 template <class T> struct VertexCtxT { v3t<T> JA, taua, taub, e; T invAri; };
@@ -314,6 +350,28 @@ I2_HD __forceinline__ q4 sing_vertex(d3 p, const VertexCtx& c) {
     r.z = -(la * c.taua.z + lb * c.taub.z);
     r.w = 2.0 * (atan2(dot(cross(ovc, c.taua), c.e), dot(c.e - ovc, c.e - c.taua)) +
                  atan2(dot(cross(ovc, c.taub), c.e), dot(c.e - ovc, c.e + c.taub)));
+    return r;
+}
+
+I2_HD __forceinline__ q4 sing_vertex_fast(d3 p, const VertexCtx& c, const i2m::MathTable* __restrict__ lt) {
+    const d3 d = p - c.JA;
+    const double l = i2m::sqrt_pos(dot(d, d));
+    const double sa = dot(c.taua, d), sb = dot(c.taub, d);
+    double na = l + sa, nb = l - sb;
+    const double s = 1e-4;
+    if (__builtin_expect(fmin(na, nb) < s * l, 0)) {
+        if (na < s * l) na = l_minus_tv(l, d, c.taua, -sa);
+        if (nb < s * l) nb = l_minus_tv(l, d, c.taub, sb);
+    }
+    const double la = i2m::log_tab(na * c.invAri, lt);
+    const double lb = i2m::log_tab(nb * c.invAri, lt);
+    q4 r;
+    r.x = -(la * c.taua.x + lb * c.taub.x);
+    r.y = -(la * c.taua.y + lb * c.taub.y);
+    r.z = -(la * c.taua.z + lb * c.taub.z);
+    const d3 le = l * c.e;
+    r.w = 2.0 * (i2m::atan2_fast(dot(cross(d, c.taua), c.e), dot(le - d, c.e - c.taua), lt) +
+                 i2m::atan2_fast(dot(cross(d, c.taub), c.e), dot(le - d, c.e + c.taub), lt));
     return r;
 }
 

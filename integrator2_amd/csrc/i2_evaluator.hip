@@ -53,7 +53,7 @@ __device__ __forceinline__ void task_ij(const TaskSource& s, int64_t t, int* i, 
 template <int TYPE> struct PairCtx;
 template <> struct PairCtx<I2_NOT_NEIGHBORS> {
     __device__ __forceinline__ void init(const Panel&, const Panel&) {}
-    __device__ __forceinline__ q4 sing(d3) const { return q4zero(); }
+    __device__ __forceinline__ q4 sing(d3, const i2m::MathTable*) const { return q4zero(); }
 };
 template <> struct PairCtx<I2_ATTACHED_NEIGHBORS> {
     EdgeCtx c;
@@ -62,7 +62,7 @@ template <> struct PairCtx<I2_ATTACHED_NEIGHBORS> {
         shifts_edge(I, J, &s1, &s2);
         c = edge_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3));
     }
-    __device__ __forceinline__ q4 sing(d3 p) const { return sing_edge(p, c); }
+    __device__ __forceinline__ q4 sing(d3 p, const i2m::MathTable* lt) const { return sing_edge_fast(p, c, lt); }
 };
 template <> struct PairCtx<I2_SIMPLE_NEIGHBORS> {
     VertexCtx c;
@@ -71,7 +71,7 @@ template <> struct PairCtx<I2_SIMPLE_NEIGHBORS> {
         shifts_vertex(I, J, &s1, &s2);
         c = vertex_ctx(vtx(J, s2), vtx(J, (s2 + 1) % 3), vtx(J, (s2 + 2) % 3), I.n, J.n, I.area);
     }
-    __device__ __forceinline__ q4 sing(d3 p) const { return sing_vertex(p, c); }
+    __device__ __forceinline__ q4 sing(d3 p, const i2m::MathTable* lt) const { return sing_vertex_fast(p, c, lt); }
 };
 
 // Quadrature over one sub-panel: area * sum_k w_k (thetaPsi - singular part)(r_k)
@@ -89,7 +89,7 @@ __device__ __forceinline__ q4 quad_sub(const Rule& R, const i2m::MathTable* lt, 
         p = p + R.L2[k] * b;
         p = p + R.L3[k] * c;
         q4 f = theta_psi_fast(p, J, lt);
-        if (TYPE != I2_NOT_NEIGHBORS) f = f - ctx.sing(p);
+        if (TYPE != I2_NOT_NEIGHBORS) f = f - ctx.sing(p, lt);
         acc = acc + R.w[k] * f;
     }
     return area * acc;

@@ -252,8 +252,8 @@ int i2_host_task_value(const double* I9, const double* J9, const int32_t* vid6, 
             p = p + l2 * b;
             p = p + l3 * cc;
             q4 f = theta_psi_fast(p, JG, &i2m::h_mathtab);
-            if (shared == 2) f = f - sing_edge(p, ec);
-            if (shared == 1) f = f - sing_vertex(p, vc);
+            if (shared == 2) f = f - sing_edge_fast(p, ec, &i2m::h_mathtab);
+            if (shared == 1) f = f - sing_vertex_fast(p, vc, &i2m::h_mathtab);
             acc = acc + w[k] * f;
         }
         total = total + area * acc;
