@@ -22,10 +22,15 @@ I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::Mat
                                             TPBar& g) {
     // ---- forward, as theta_psi_fast
     const d3 da = p - J.a, db = p - J.b, dc = p - J.c;
-    const double la = i2m::sqrt_pos(dot(da, da)), lb = i2m::sqrt_pos(dot(db, db)), lc = i2m::sqrt_pos(dot(dc, dc));
-    double n1, e1, n2, e2, n3, e3;
-    bool f1, f2, f3;
-    theta_psi_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
+    const double qa = dot(da, da), qb = dot(db, db), qc = dot(dc, dc);
+    const double la = i2m::sqrt_pos(qa), lb = i2m::sqrt_pos(qb), lc = i2m::sqrt_pos(qc);
+    const double s1 = dot(da, J.tc), s2 = dot(db, J.ta), s3 = dot(dc, J.tb);
+    double n1 = la + s1, e1 = lb + dot(db, J.tc);
+    double n2 = lb + s2, e2 = lc + dot(dc, J.ta);
+    double n3 = lc + s3, e3 = la + dot(da, J.tb);
+    bool f1 = false, f2 = false, f3 = false;
+    if (__builtin_expect(terms_need_fix(la, lb, lc, n1, e1, n2, e2, n3, e3), 0))
+        fix_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
     const double e12 = e1 * e2;
     const double inv = i2m::rcp_pos(e12 * e3);
     const double t1 = i2m::log_tab(n1 * (e2 * e3) * inv, lt);
@@ -35,15 +40,18 @@ I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::Mat
     r.x = t1 * J.tc.x + t2 * J.ta.x + t3 * J.tb.x;
     r.y = t1 * J.tc.y + t2 * J.ta.y + t3 * J.tb.y;
     r.z = t1 * J.tc.z + t2 * J.ta.z + t3 * J.tb.z;
+    // the solid angle from da, db, dc directly: with the tangents and vertices
+    // as free parameters of this function, x and y then depend on da, db, dc only
+    const double pab = dot(da, db), pbc = dot(db, dc), pca = dot(dc, da);
     const double y = dot(cross(da, db), dc);
-    const double x = la * lb * lc + dot(da, db) * lc + dot(db, dc) * la + dot(dc, da) * lb;
+    const double x = la * lb * lc + pab * lc + pbc * la + pca * lb;
     r.w = 2.0 * i2m::atan2_fast(y, x, lt);
     // ---- reverse
     const d3 gP = {gb.x, gb.y, gb.z};
     const double tb1 = dot(gP, J.tc), tb2 = dot(gP, J.ta), tb3 = dot(gP, J.tb);
     axpy(g.tc, t1, gP); axpy(g.ta, t2, gP); axpy(g.tb, t3, gP);
-    // t_k = log n_k - log e_k: reciprocals from two divisions
-    const double invn = 1.0 / (n1 * n2 * n3);
+    // t_k = log n_k - log e_k: reciprocals from two refined reciprocals
+    const double invn = i2m::rcp_pos(n1 * n2 * n3);
     const double rn1 = n2 * n3 * invn, rn2 = n1 * n3 * invn, rn3 = n1 * n2 * invn;
     const double re1 = e2 * e3 * inv, re2 = e1 * e3 * inv, re3 = e12 * inv;
     const double nb1 = tb1 * rn1, eb1 = -tb1 * re1;
@@ -58,17 +66,18 @@ I2_HD __forceinline__ q4 theta_psi_fast_rev(d3 p, const JGeom& J, const i2m::Mat
     if (!f3) { lcb += nb3; axpy(dcb, nb3, J.tb); axpy(g.tb, nb3, dc); lab += eb3; axpy(dab, eb3, J.tb); axpy(g.tb, eb3, da); }
     else { lab += nb3; lcb += eb3; }
     // Theta = 2 atan2(y, x)
-    const double r2 = 1.0 / (x * x + y * y);
+    // (the derivative of y = da.(db x dc) in da, db, dc: the cross products)
+    const double r2 = i2m::rcp_pos(x * x + y * y);
     const double yb = 2.0 * gb.w * x * r2, xb = -2.0 * gb.w * y * r2;
     axpy(dab, yb, cross(db, dc)); axpy(dbb, yb, cross(dc, da)); axpy(dcb, yb, cross(da, db));
-    lab += xb * (lb * lc + dot(db, dc));
-    lbb += xb * (la * lc + dot(dc, da));
-    lcb += xb * (la * lb + dot(da, db));
+    lab += xb * (lb * lc + pbc);
+    lbb += xb * (la * lc + pca);
+    lcb += xb * (la * lb + pab);
     axpy(dab, xb * lc, db); axpy(dab, xb * lb, dc);
     axpy(dbb, xb * lc, da); axpy(dbb, xb * la, dc);
     axpy(dcb, xb * la, db); axpy(dcb, xb * lb, da);
     // l = |d|
-    const double il = 1.0 / (la * lb * lc);
+    const double il = i2m::rcp_pos(la * lb * lc);
     axpy(dab, lab * (lb * lc * il), da);
     axpy(dbb, lbb * (la * lc * il), db);
     axpy(dcb, lcb * (la * lb * il), dc);

@@ -186,17 +186,6 @@ I2_HD __forceinline__ bool terms_need_fix(double la, double lb, double lc, doubl
     const double s = 1e-4;
     return fmin(fmin(fmin(n1, e3) - s * la, fmin(e1, n2) - s * lb), fmin(e2, n3) - s * lc) < 0.0;
 }
-// the six l + d.tau terms of the inner integral, each from its own dot product
-I2_HD __forceinline__ void theta_psi_terms(const JGeom& J, d3 da, d3 db, d3 dc, double la, double lb, double lc,
-                                           double& n1, double& e1, double& n2, double& e2, double& n3, double& e3,
-                                           bool& f1, bool& f2, bool& f3) {
-    n1 = la + dot(da, J.tc); e1 = lb + dot(db, J.tc);
-    n2 = lb + dot(db, J.ta); e2 = lc + dot(dc, J.ta);
-    n3 = lc + dot(dc, J.tb); e3 = la + dot(da, J.tb);
-    f1 = f2 = f3 = false;
-    if (__builtin_expect(terms_need_fix(la, lb, lc, n1, e1, n2, e2, n3, e3), 0))
-        fix_terms(J, da, db, dc, la, lb, lc, n1, e1, n2, e2, n3, e3, f1, f2, f3);
-}
 // The forward kernels reuse three of the terms' dot products in the solid
 // angle: with db = da - Lc tc (and cyclically), da.db = |da|^2 - Lc da.tc,
 // db.dc = |db|^2 - La db.ta, dc.da = |dc|^2 - Lb dc.tb; and
