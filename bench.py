@@ -61,8 +61,8 @@ POINTS = 13
 EXECUTED_FLOPS_PER_POINT = 274
 # HBM bytes per sweep dispatch on this workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE and scaled by a 1 GiB copy's readings
-# (545 MB read + 2.62 GB written; the algorithmic figure is 1.70 GB)
-SWEEP_HBM_BYTES = 3.17e9
+# (644 MB read + 2.63 GB written; the algorithmic figure is 1.70 GB)
+SWEEP_HBM_BYTES = 3.27e9
 
 
 def flops_per_point():
