@@ -91,6 +91,23 @@ __device__ __forceinline__ int reported_level(const AdjArgs& A, int c, int K) {
     if (A.quirk && c >= 1 && ((K - c) & 1)) return c - 1;
     return c;
 }
+// A wavefront-uniform double or panel, moved to scalar registers
+__device__ __forceinline__ double uniform_d(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ d3 uniform_d3(d3 v) { return {uniform_d(v.x), uniform_d(v.y), uniform_d(v.z)}; }
+__device__ __forceinline__ Panel uniform_panel(const Panel& P) {
+    Panel U;
+    U.a = uniform_d3(P.a); U.b = uniform_d3(P.b); U.c = uniform_d3(P.c);
+    U.ta = uniform_d3(P.ta); U.tb = uniform_d3(P.tb); U.tc = uniform_d3(P.tc);
+    U.n = uniform_d3(P.n); U.area = uniform_d(P.area);
+    U.v0 = __builtin_amdgcn_readfirstlane(P.v0); U.v1 = __builtin_amdgcn_readfirstlane(P.v1);
+    U.v2 = __builtin_amdgcn_readfirstlane(P.v2); U.pad = 0;
+    return U;
+}
 __device__ __forceinline__ double wsum(double x) {
     for (int m = 1; m < 64; m <<= 1) x += __shfl_xor(x, m);
     return x;
@@ -153,7 +170,7 @@ __device__ __forceinline__ void child_bary(int ch, d3& ba, d3& bb, d3& bc) {
 // ------------------------------------------------------------ disjoint sweep
 constexpr int ADJ_BLOCK = 256;
 
-__global__ __launch_bounds__(ADJ_BLOCK) void k_adj_disjoint(AdjArgs A) {
+__global__ __launch_bounds__(ADJ_BLOCK) I2_TWO_WAVES void k_adj_disjoint(AdjArgs A) {
     __shared__ RuleSh R;
     __shared__ long long item_sh;
     load_rule_sh(R, A.rule);
@@ -179,7 +196,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_disjoint(AdjArgs A) {
         double Sb = 0.0, loss = 0.0;
         const int j0 = jt * JT, j1 = min(M.nT, j0 + JT);
         for (int j = j0; j < j1; ++j) {
-            const Panel J = M.panels[j];
+            const Panel J = uniform_panel(M.panels[j]);   // j is uniform: K_j lives in scalar registers
             bool valid = ivalid && j != i && shared_count(I, J) == 0;
             int lev = 0;
             if (valid) {
@@ -270,8 +287,8 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_deep(AdjArgs A) {
          t += (unsigned long long)gridDim.x * waves) {
         const AdjEntry e = A.deep[t];
         const AdjMesh& M = A.meshes[e.m];
-        const Panel I = M.panels[e.i];
-        const Panel J = M.panels[e.j];
+        const Panel I = uniform_panel(M.panels[e.i]);   // one wavefront per pair
+        const Panel J = uniform_panel(M.panels[e.j]);
         const Weights W = weights(A.seed, e.i, e.j, M.nT);
         const q4 gb = task_cotangent(W, J.n);
         const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
@@ -345,8 +362,8 @@ struct SingBars {
 template <int TYPE>
 __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R, const AdjMesh& M, int i, int j, int level,
                                               int lane) {
-    const Panel I = M.panels[i];
-    const Panel J = M.panels[j];
+    const Panel I = uniform_panel(M.panels[i]);   // one wavefront per pair
+    const Panel J = uniform_panel(M.panels[j]);
     const Weights W = weights(A.seed, i, j, M.nT);
     const q4 gb = task_cotangent(W, J.n);
     const JGeom JG = make_jgeom(J.a, J.b, J.c, J.ta, J.tb, J.tc);
