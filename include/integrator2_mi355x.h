@@ -96,12 +96,24 @@ int i2_mesh_copy_measures(const i2_mesh* m, double* s);
 int i2_mesh_copy_neighbors(const i2_mesh* m, int type, int32_t* ijk);
 /* Device pointer and length of a mesh array for deviceVector-style getters:
  * what = 0 vertices (double3), 1 cells (int3), 2 normals (double3),
- * 3 measures (double), 4 simple, 5 attached neighbours (int3). */
+ * 3 measures (double), 4 simple, 5 attached, 6 disjoint neighbours (int3;
+ * the disjoint list is written out on the first request, I2_ERR_RANGE beyond
+ * 2^31 - 1 pairs). */
 int i2_mesh_device_array(const i2_mesh* m, int what, const void** dptr, int64_t* n);
+/* exportMeshToObj (src/Mesh3d.cu:264-276): OBJ text with 1-based faces. */
+int i2_export_mesh_obj(const char* path, const double* xyz, int64_t nV, const int32_t* tri, int64_t nT);
+/* exportMeshToVtk (src/Mesh3d.cu:278-343): VTK XML PolyData (ASCII) with one
+ * Int32 cell array "<type>Refinements" per non-NULL count array (nT bytes). */
+int i2_export_mesh_vtk(const char* path, const double* xyz, int64_t nV, const int32_t* tri, int64_t nT,
+                       const uint8_t* simpleRefinements, const uint8_t* attachedRefinements, const uint8_t* notRefinements);
 
 /* ======================================================================
  * NumericalIntegrator3D (src/NumericalIntegrator3d.cuh:89-327)
  * ==================================================================== */
+/* The triangle rules qf3D1, qf3D3, qf3D4, qf3D6, qf3D7, qf3D9, qf3D12, qf3D13
+ * (src/QuadratureFormula3d.cuh:34-213, same decimal literals): nPts (L1, L2)
+ * pairs into L12, nPts weights into w, the order into *order (any may be NULL). */
+int i2_quadrature_rule(int nPts, double* L12, double* w, int* order);
 /* NumericalIntegrator3D(mesh, qf) (src/NumericalIntegrator3d.cu:197-216):
  * L12 = nPts (L1, L2) pairs, w = nPts weights, order = rule order (the Runge
  * factor is 2^order).  Default mode: automatic error control with the
@@ -115,6 +127,28 @@ int i2_integrator_set_fixed_refinement_level(i2_integrator* it, int level);
  * MAX_REFINE_LEVEL / EPS_INTEGRATION in the reference).  refParityQuirk != 0
  * reproduces the reference's result-buffer swap (src/evaluators/evaluatorJ3DK.cu:976-990). */
 int i2_integrator_set_error_control(i2_integrator* it, int maxLevel, double tol, int refParityQuirk);
+/* NumericalIntegrator3D::getRefinementsRequired (src/NumericalIntegrator3d.cuh:316-324):
+ * per original cell, how many refinement passes of the last automatic run
+ * on this integrator refined it for `type` (the counts of
+ * determineCellsToBeRefined, src/NumericalIntegrator3d.cu:461-499: a task
+ * converged at iteration c counts c, one unconverged after the comparison at
+ * maxLevel counts maxLevel + 1; a cell takes the largest over its tasks).
+ * *n = nT, or 0 in fixed-level mode or before a run (the reference allocates
+ * the counts for automatic error control only).  d_counts receives a device
+ * pointer valid until the next run; h_counts (nT bytes) a host copy; either
+ * may be NULL.  The evaluator of that run must still exist. */
+int i2_integrator_refinements_required(i2_integrator* it, int type, const void** d_counts, uint8_t* h_counts, int64_t* n);
+/* getRefinedVertices / getRefinedCells / getRefinedCellMeasures
+ * (src/NumericalIntegrator3d.cuh:243-274) in fixed-level mode: the mesh split
+ * L times by kSplitCell (src/NumericalIntegrator3d.cu:37-87): nV + nT (4^L - 1)
+ * vertices (double3) and nT 4^L cells (int3) with measures (double).  The
+ * slots of the new vertices and cells of cell idx are fixed at 3 idx and 4 idx
+ * of each pass (the reference takes them from atomic counters).  Device
+ * pointers owned by the integrator; built on the first call per (mesh, level).
+ * I2_ERR_STATE in automatic mode (the reference's arrays then hold the
+ * working state of the last refined type). */
+int i2_integrator_refined_mesh(i2_integrator* it, const void** d_vertices, const void** d_cells, const void** d_measures,
+                               int64_t* nV, int64_t* nT);
 
 /* ======================================================================
  * Evaluator3D / EvaluatorJ3DK (src/evaluators/evaluator3d.cuh:42-100,
@@ -132,6 +166,11 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness);
 int i2_evaluator_run_pairs(i2_evaluator* ev, const int32_t* simple3, int64_t nSimple,
                            const int32_t* attached3, int64_t nAttached, const int32_t* not3, int64_t nNot);
 int i2_evaluator_task_count(const i2_evaluator* ev, int type, int64_t* n);
+/* Evaluator3D::getTasks (src/evaluators/evaluator3d.cuh:89-101): device
+ * pointer to the (i, j, k) list of a type (int3, *n entries).  The disjoint
+ * list of runAllPairs is decoded on the fly by the kernels; this call writes
+ * it out once per run (I2_ERR_RANGE beyond 2^31 - 1 tasks). */
+int i2_evaluator_device_tasks(i2_evaluator* ev, int type, const void** d_tasks, int64_t* n);
 /* Results of the last run for one type, in task order; any output may be NULL.
  * tasks3: (i, j, k); integrals4: (Psi_x, Psi_y, Psi_z, Theta); results3: J;
  * errors: symmetry error (only after runAllPairs with checkCorrectness);
@@ -155,6 +194,9 @@ int i2_evaluator_launch_records(const i2_evaluator* ev, int64_t capacity, int32_
 int i2_evaluator_sweep_ms(const i2_evaluator* ev, double* ms);
 /* hipDeviceSynchronize on `device`. */
 int i2_device_synchronize(int device);
+/* copy_d2h (src/common/cuda_memory.cuh:104-109) for the device pointers the
+ * getters return: `bytes` from device memory `src` to host memory `dst`. */
+int i2_copy_to_host(void* dst, const void* src, int64_t bytes);
 /* Evaluator3D::outputResultsToFile (src/evaluators/evaluator3d.cu:354-457):
  * writes SimpleNeighbors/AttachedNeighbors/NotNeighbors .dat or .csv into dir
  * (NULL = current directory). */

@@ -110,6 +110,42 @@ def torus(nmajor=256, nminor=128, R=1.0, r=0.25, jitter=0.0, seed=0):
     return xyz, tri
 
 
+def quadrature_rule(nPts):
+    """The reference's triangle rule with nPts points (qf3D1 ... qf3D13,
+    src/QuadratureFormula3d.cuh): (L12 of shape (nPts, 2), weights, order)."""
+    L12 = np.zeros((nPts, 2)) if nPts > 0 else np.zeros((0, 2))
+    w = np.zeros(max(nPts, 0))
+    order = ctypes.c_int()
+    _check(lib().i2_quadrature_rule(ctypes.c_int(nPts), _ptr(L12), _ptr(w), ctypes.byref(order)))
+    return L12, w, order.value
+
+
+def export_mesh_obj(filename, xyz, tri):
+    """exportMeshToObj (src/Mesh3d.cu:264-276)."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    tri = np.ascontiguousarray(tri, np.int32)
+    _check(lib().i2_export_mesh_obj(filename.encode(), _ptr(xyz), _I64(len(xyz)), _ptr(tri), _I64(len(tri))))
+
+
+def export_mesh_vtk(filename, xyz, tri, refinements=(None, None, None)):
+    """exportMeshToVtk (src/Mesh3d.cu:278-343); refinements: three per-cell count arrays or None."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    tri = np.ascontiguousarray(tri, np.int32)
+    refs = [None if r is None or len(r) == 0 else np.ascontiguousarray(r, np.uint8) for r in refinements]
+    for r in refs:
+        if r is not None and len(r) != len(tri):
+            raise I2Error("a refinement-count array needs one entry per cell")
+    _check(lib().i2_export_mesh_vtk(filename.encode(), _ptr(xyz), _I64(len(xyz)), _ptr(tri), _I64(len(tri)),
+                                    _ptr(refs[0]), _ptr(refs[1]), _ptr(refs[2])))
+
+
+def _device_to_host(dptr, shape, dtype):
+    out = np.zeros(shape, dtype)
+    if out.size:
+        _check(lib().i2_copy_to_host(_ptr(out), dptr, _I64(out.nbytes)))
+    return out
+
+
 def batch_spec(nmeshes=64, seed=0):
     levels = np.zeros(nmeshes, np.int32)
     axes = np.zeros((nmeshes, 3))
@@ -209,6 +245,25 @@ class NumericalIntegrator3D:
     def setErrorControl(self, maxLevel=5, tol=1e-5, refParityQuirk=True):
         _check(lib().i2_integrator_set_error_control(self._h, ctypes.c_int(maxLevel), _D(tol), ctypes.c_int(int(refParityQuirk))))
 
+    def getRefinementsRequired(self, t):
+        """Per original cell, the refinement passes of the last automatic run for type t
+        (empty in fixed-level mode), as uint8."""
+        n = _I64()
+        _check(lib().i2_integrator_refinements_required(self._h, ctypes.c_int(t), None, None, ctypes.byref(n)))
+        out = np.zeros(n.value, np.uint8)
+        if n.value:
+            _check(lib().i2_integrator_refinements_required(self._h, ctypes.c_int(t), None, _ptr(out), ctypes.byref(n)))
+        return out
+
+    def refinedMesh(self):
+        """(vertices (nV, 3), cells (nT, 3), measures (nT,)) of the mesh split to the fixed level."""
+        dv, dc, dm = _P(), _P(), _P()
+        nV, nT = _I64(), _I64()
+        _check(lib().i2_integrator_refined_mesh(self._h, ctypes.byref(dv), ctypes.byref(dc), ctypes.byref(dm),
+                                                ctypes.byref(nV), ctypes.byref(nT)))
+        return (_device_to_host(dv, (nV.value, 3), np.float64), _device_to_host(dc, (nT.value, 3), np.int32),
+                _device_to_host(dm, (nT.value,), np.float64))
+
 
 # ---------------------------------------------------------------- EvaluatorJ3DK
 class EvaluatorJ3DK:
@@ -236,6 +291,13 @@ class EvaluatorJ3DK:
         n = _I64()
         _check(lib().i2_evaluator_task_count(self._h, ctypes.c_int(t), ctypes.byref(n)))
         return n.value
+
+    def getTasks(self, t):
+        """Host copy of the device (i, j, k) task list of type t (Evaluator3D::getTasks)."""
+        d = _P()
+        n = _I64()
+        _check(lib().i2_evaluator_device_tasks(self._h, ctypes.c_int(t), ctypes.byref(d), ctypes.byref(n)))
+        return _device_to_host(d, (n.value, 3), np.int32)
 
     def results(self, t, errors=False):
         """Dict with tasks (n,3), integrals (n,4), results (n,3), conv (n,), errors (n,) if requested."""

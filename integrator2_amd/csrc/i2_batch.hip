@@ -182,6 +182,7 @@ struct StepOut {
     bool pushed;
     int c;
     q4 cur, prev;
+    bool unconv = false;   // finished at maxLevel with the Runge criterion above the tolerance
 };
 
 __device__ __forceinline__ void push_deep(const BatchArgs& A, int level, int type, int m, int i, int j, int64_t z, q4 prev) {
@@ -197,7 +198,7 @@ __device__ __forceinline__ void push_deep(const BatchArgs& A, int level, int typ
 // Writes a finished task: per-pair arrays, or its contribution to the row sums
 // of both parities of K (returned in rp0/rp1 as (J, Theta)).
 template <int TYPE, int OUT>
-__device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& M, int64_t z, const StepOut& o,
+__device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& M, int64_t z, int i, const StepOut& o,
                                             const Panel& I, const Panel& J, q4* rp0, q4* rp1) {
     q4 cur = o.cur, prev = o.prev;
     if (TYPE != I2_NOT_NEIGHBORS) {
@@ -209,6 +210,7 @@ __device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& 
         M.fin[TYPE][z] = make_double4(cur.x, cur.y, cur.z, cur.w);
         M.prev[TYPE][z] = make_double4(prev.x, prev.y, prev.z, prev.w);
         M.conv[TYPE][z] = (int8_t)o.c;
+        if (o.unconv && M.unconv[TYPE]) M.unconv[TYPE][i] = 1;
     } else {
         if (M.conv[TYPE]) M.conv[TYPE][z] = (int8_t)o.c;   // kept for the adjoint
         // K even selects I_c for even c, I_{c-1} for odd c; K odd the opposite
@@ -271,7 +273,7 @@ __device__ __forceinline__ StepOut sweep_task(const BatchArgs& A, const RuleLDS&
     }
     if (A.last_level == 0) { o.finished = true; o.c = 0; o.cur = o.prev = v0; return o; }
     const double crit = runge_criterion(v1, v0, A.rule.pow2p);
-    if (!(crit > A.tol) || A.last_level == 1) { o.finished = true; o.c = 1; o.cur = v1; o.prev = A.quirk ? v0 : v1; }
+    if (!(crit > A.tol) || A.last_level == 1) { o.finished = true; o.c = 1; o.cur = v1; o.prev = A.quirk ? v0 : v1; o.unconv = crit > A.tol; }
     else { push_deep(A, 2, TYPE, m, i, j, z, v1); o.pushed = true; }
     return o;
 }
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(256) I2_THREE_WAVES void k_sweep_disjoint(BatchArgs
         const int64_t z = valid ? disjoint_task(M, i, j) : 0;
         const StepOut o = sweep_task<I2_NOT_NEIGHBORS, LPP>(A, R, m, i, j, z, valid, s, I, J, &pts);
         q4 rp0 = q4zero(), rp1 = q4zero();
-        if (o.finished) finish_task<I2_NOT_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
+        if (o.finished) finish_task<I2_NOT_NEIGHBORS, OUT>(A, M, z, i, o, I, J, &rp0, &rp1);
         if (OUT == OUT_ROW_SUMS) {
             // i is uniform: reduce over the wavefront (or the whole smaller
             // workgroup), one set of atomics per wavefront
@@ -356,13 +358,13 @@ __global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_singular(BatchArgs A
         if (type == I2_SIMPLE_NEIGHBORS) {
             const StepOut o = sweep_task<I2_SIMPLE_NEIGHBORS, 1>(A, R, m, i, j, z, valid, 0, I, J);
             if (o.finished) {
-                finish_task<I2_SIMPLE_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
+                finish_task<I2_SIMPLE_NEIGHBORS, OUT>(A, M, z, i, o, I, J, &rp0, &rp1);
                 if (OUT == OUT_ROW_SUMS) add_row(M, type, i, rp0, rp1);
             }
         } else {
             const StepOut o = sweep_task<I2_ATTACHED_NEIGHBORS, 1>(A, R, m, i, j, z, valid, 0, I, J);
             if (o.finished) {
-                finish_task<I2_ATTACHED_NEIGHBORS, OUT>(A, M, z, o, I, J, &rp0, &rp1);
+                finish_task<I2_ATTACHED_NEIGHBORS, OUT>(A, M, z, i, o, I, J, &rp0, &rp1);
                 if (OUT == OUT_ROW_SUMS) add_row(M, type, i, rp0, rp1);
             }
         }
@@ -397,12 +399,12 @@ __device__ __forceinline__ void deep_task(const BatchArgs& A, const RuleLDS& R, 
         o.finished = true;   // fixed mode queues tasks straight at the fixed level
     } else {
         const double crit = runge_criterion(acc, prev, A.rule.pow2p);
-        if (!(crit > A.tol) || level == A.last_level) { o.finished = true; o.prev = A.quirk ? prev : acc; }
+        if (!(crit > A.tol) || level == A.last_level) { o.finished = true; o.prev = A.quirk ? prev : acc; o.unconv = crit > A.tol; }
         else push_deep(A, level + 1, TYPE, e.m, e.i, e.j, e.z, acc);
     }
     if (o.finished) {
         q4 rp0, rp1;
-        finish_task<TYPE, OUT>(A, M, e.z, o, I, J, &rp0, &rp1);
+        finish_task<TYPE, OUT>(A, M, e.z, e.i, o, I, J, &rp0, &rp1);
         if (OUT == OUT_ROW_SUMS) add_row(M, TYPE, e.i, rp0, rp1);
     }
 }
@@ -784,6 +786,7 @@ struct i2_batch {
         i2::DBuf<double4> fin[3], prev[3];
         i2::DBuf<int8_t> conv[3];
         i2::DBuf<double> res[3];
+        i2::DBuf<unsigned long long> unconv;   // per type and control panel: unconverged at maxLevel (bytes)
         i2::DBuf<double> rowacc, rowout;
     };
     std::vector<Out> outs;
@@ -840,6 +843,8 @@ static int batch_prepare(i2_batch* b) {
                 I2_TRY(o.fin[t].ensure(n)); I2_TRY(o.prev[t].ensure(n)); I2_TRY(o.conv[t].ensure(n)); I2_TRY(o.res[t].ensure(3 * n));
                 D.fin[t] = o.fin[t].p; D.prev[t] = o.prev[t].p; D.conv[t] = o.conv[t].p; D.res[t] = o.res[t].p;
             }
+            I2_TRY(o.unconv.ensure((3 * mesh->nT + 7) / 8));
+            for (int t = 0; t < 3; ++t) D.unconv[t] = reinterpret_cast<uint8_t*>(o.unconv.p) + t * mesh->nT;
         } else {
             I2_TRY(o.rowacc.ensure(24 * mesh->nT));
             I2_TRY(o.rowout.ensure(4 * mesh->nT));
@@ -874,6 +879,11 @@ static int batch_prepare(i2_batch* b) {
 
 static int batch_enqueue_all(i2_batch* b) {
     Engine& E = b->E;
+    if (E.out == OUT_PER_PAIR && !b->cfg.fixed)
+        for (auto& o : b->outs) {
+            hipLaunchKernelGGL(k_zero_u64, dim3(64), dim3(256), 0, b->stream, o.unconv.p, (int64_t)o.unconv.n);
+            I2_TRY(check_launch("k_zero_u64"));
+        }
     if (E.out == OUT_ROW_SUMS)
         for (auto& o : b->outs) {
             hipLaunchKernelGGL(k_zero_f64, dim3(1024), dim3(256), 0, b->stream, o.rowacc.p, (int64_t)o.rowacc.n);
@@ -995,6 +1005,11 @@ void engine_evaluator_outputs(const i2_evaluator* ev, int type, const double4** 
     *ints = b->outs[0].fin[type].p;
     *res = b->outs[0].res[type].p;
     *conv = b->outs[0].conv[type].p;
+}
+
+const uint8_t* engine_evaluator_unconverged(const i2_evaluator* ev, int type) {
+    const i2_batch* b = ev->fast;
+    return reinterpret_cast<const uint8_t*>(b->outs[0].unconv.p) + type * ev->mesh->nT;
 }
 
 void engine_release_evaluator(i2_evaluator* ev) {

@@ -23,6 +23,9 @@ struct MeshDesc {
     double4* prev[3];
     int8_t* conv[3];
     double* res[3];
+    // per control panel: 1 where a task was still unconverged after the
+    // comparison at maxLevel (automatic mode; nT bytes per type, may be null)
+    uint8_t* unconv[3];
     // row-sum outputs: rowacc[type][parity][nT][4], rowout[nT][4]
     double* rowacc;
     double* rowout;
@@ -109,5 +112,7 @@ int engine_run_evaluator(i2_evaluator* ev);
 void engine_release_evaluator(i2_evaluator* ev);
 // Device pointers of the per-pair outputs of the evaluator's fast path.
 void engine_evaluator_outputs(const i2_evaluator* ev, int type, const double4** ints, const double** res, const int8_t** conv);
+// Per control panel of `type`: 1 where a task was unconverged after the comparison at maxLevel.
+const uint8_t* engine_evaluator_unconverged(const i2_evaluator* ev, int type);
 
 }  // namespace i2

@@ -20,35 +20,6 @@
 namespace i2 {
 using namespace i2d;
 
-// ------------------------------------------------------------ task decoding
-__device__ __forceinline__ int find_row(const int64_t* __restrict__ row_start, int nT, int64_t k) {
-    int lo = 0, hi = nT - 1;   // largest i with row_start[i] <= k
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (row_start[mid] <= k) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-// k-th disjoint (i<j) pair in lexicographic order
-__device__ __forceinline__ void decode_disjoint(const TaskSource& s, int64_t k, int* pi, int* pj) {
-    const int i = find_row(s.row_start, s.nT, k);
-    int64_t j = i + 1 + (k - s.row_start[i]);
-    for (int64_t q = s.nbr_start[i]; q < s.nbr_start[i + 1]; ++q) {
-        if (s.nbr[q] <= j) ++j; else break;
-    }
-    *pi = i; *pj = (int)j;
-}
-template <bool IMPLICIT>
-__device__ __forceinline__ void task_ij(const TaskSource& s, int64_t t, int* i, int* j) {
-    if (IMPLICIT) {
-        if (t < s.n_unordered) decode_disjoint(s, t, i, j);
-        else decode_disjoint(s, t - s.n_unordered, j, i);
-    } else {
-        *i = s.ij3[3 * t];
-        *j = s.ij3[3 * t + 1];
-    }
-}
-
 // ------------------------------------------------------------ per-pair integrand
 template <int TYPE> struct PairCtx;
 template <> struct PairCtx<I2_NOT_NEIGHBORS> {
@@ -272,7 +243,9 @@ static int run_type(i2_evaluator* ev, TypeState& ts, const TaskSource& src) {
     } else {
         I2_TRY(ts.rest[0].ensure(n));
         I2_TRY(ts.rest[1].ensure(n));
+        I2_TRY(ts.unconv.ensure(src.nT));
         I2_HIP(hipMemsetAsync(ts.conv.p, 0, (size_t)n, ev->stream));
+        I2_HIP(hipMemsetAsync(ts.unconv.p, 0, (size_t)src.nT, ev->stream));
         I2_TRY((launch_level<TYPE, IMPLICIT>(ev, n, nullptr, src, 0, ts.buf[0].p)));
         ts.refined_tasks = n;
         int64_t remaining = n;
@@ -295,6 +268,8 @@ static int run_type(i2_evaluator* ev, TypeState& ts, const TaskSource& src) {
         }
         ts.K = iter;
         ts.final_buf = iter % 2;
+        if (remaining > 0)   // still unconverged after the comparison at maxLevel (all tasks when maxLevel = 0)
+            I2_TRY(mark_unconverged(ev, IMPLICIT, src, remaining, iter == 0 ? nullptr : ts.rest[cur_rest].p, ts.unconv.p));
     }
     hipLaunchKernelGGL((k_finalize<TYPE, IMPLICIT>), dim3(blocks_for(n, 256)), dim3(256), 0, ev->stream, n, src,
                        ev->mesh->d_panels.p, ts.buf[ts.final_buf].p, ts.results.p);
@@ -307,6 +282,22 @@ static int run_type(i2_evaluator* ev, TypeState& ts, const TaskSource& src) {
     return I2_OK;
 }
 
+TaskSource task_source(const i2_evaluator* ev, int type) {
+    const i2_mesh* m = ev->mesh;
+    const TypeState& ts = ev->type[type];
+    TaskSource src;
+    src.nT = (int)m->nT;
+    if (ts.implicit) {
+        src.n_unordered = m->pairs[I2_NOT_NEIGHBORS];
+        src.row_start = m->d_row_start.p;
+        src.nbr_start = m->d_nbr_start.p;
+        src.nbr = m->d_nbr.p;
+    } else {
+        src.ij3 = ts.tasks.p;
+    }
+    return src;
+}
+
 int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check) {
     const i2_mesh* m = ev->mesh;
     DeviceGuard g(m->device);
@@ -317,17 +308,12 @@ int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check) {
     // types in the reference's order: vertex-sharing, edge-sharing, disjoint (evaluator3d.cu:173-186)
     for (int t = 0; t < 3; ++t) {
         TypeState& ts = ev->type[t];
-        TaskSource src;
-        src.nT = (int)m->nT;
+        const TaskSource src = task_source(ev, t);
         ts.has_errors = false;
+        ts.refinements_ready = false;
         if (ts.implicit) {
-            src.n_unordered = m->pairs[I2_NOT_NEIGHBORS];
-            src.row_start = m->d_row_start.p;
-            src.nbr_start = m->d_nbr_start.p;
-            src.nbr = m->d_nbr.p;
             I2_TRY((run_type<I2_NOT_NEIGHBORS, true>(ev, ts, src)));
         } else {
-            src.ij3 = ts.tasks.p;
             if (t == I2_SIMPLE_NEIGHBORS) I2_TRY((run_type<I2_SIMPLE_NEIGHBORS, false>(ev, ts, src)));
             else if (t == I2_ATTACHED_NEIGHBORS) I2_TRY((run_type<I2_ATTACHED_NEIGHBORS, false>(ev, ts, src)));
             else I2_TRY((run_type<I2_NOT_NEIGHBORS, false>(ev, ts, src)));
@@ -342,6 +328,7 @@ int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check) {
     }
     I2_HIP(hipStreamSynchronize(ev->stream));
     ev->ran = true;
+    ev->integ->last_ev = ev;
     return I2_OK;
 }
 
@@ -360,6 +347,7 @@ static std::vector<int32_t> with_reversed(const std::vector<int32_t>& l) {
 static int set_tasks(i2_evaluator* ev, int t, const std::vector<int32_t>& h) {
     TypeState& ts = ev->type[t];
     ts.implicit = false;
+    ts.task_list_ready = false;
     ts.n = (int64_t)h.size() / 3;
     I2_TRY(ts.tasks.ensure(3 * ts.n));
     if (ts.n) I2_HIP(hipMemcpy(ts.tasks.p, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice));
@@ -394,7 +382,11 @@ int i2_integrator_create(i2_integrator** out, const i2_mesh* mesh, const double*
     *out = it;
     return I2_OK;
 }
-void i2_integrator_destroy(i2_integrator* it) { delete it; }
+void i2_integrator_destroy(i2_integrator* it) {
+    if (!it) return;
+    release_refined_mesh(it);
+    delete it;
+}
 
 int i2_integrator_set_fixed_refinement_level(i2_integrator* it, int level) {
     if (!it) { set_error("null integrator"); return I2_ERR_INVALID; }
@@ -444,7 +436,9 @@ void i2_evaluator_destroy(i2_evaluator* ev) {
         for (auto& ts : ev->type) {
             ts.tasks.reset(); ts.buf[0].reset(); ts.buf[1].reset(); ts.results.reset(); ts.errors.reset();
             ts.conv.reset(); ts.rest[0].reset(); ts.rest[1].reset();
+            ts.unconv.reset(); ts.refinements.reset(); ts.task_list.reset();
         }
+        if (ev->integ && ev->integ->last_ev == ev) ev->integ->last_ev = nullptr;
         clear_launches(ev);
         engine_release_evaluator(ev);
         adjoint_release(ev->adj);
@@ -466,6 +460,7 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
     I2_TRY(set_tasks(ev, I2_ATTACHED_NEIGHBORS, with_reversed(m->h_attached)));
     TypeState& d = ev->type[I2_NOT_NEIGHBORS];
     d.implicit = true;
+    d.task_list_ready = false;
     d.n = 2 * m->pairs[I2_NOT_NEIGHBORS];
     d.tasks.reset();
     if (ev->path == 0) return evaluator_run(ev, true, checkCorrectness != 0);
@@ -477,6 +472,7 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
     for (int t = 0; t < 3; ++t) {
         TypeState& ts = ev->type[t];
         ts.has_errors = false;
+        ts.refinements_ready = false;
         if (checkCorrectness && ts.n > 0) {
             const double4* ints;
             const double* res;
@@ -491,6 +487,7 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
     }
     I2_HIP(hipStreamSynchronize(ev->stream));
     ev->ran = true;
+    ev->integ->last_ev = ev;
     return I2_OK;
 }
 

@@ -84,6 +84,7 @@ struct i2_mesh {
     int device = 0;
     hipStream_t stream = nullptr;
     bool prepared = false;
+    uint64_t generation = 0;            // incremented by every prepare (caches key on it)
     int64_t nV = 0, nT = 0;
     std::vector<double> h_xyz;          // host copy of vertices (AoS)
     std::vector<int32_t> h_tri;         // host copy of cells (AoS, 0-based)
@@ -103,8 +104,15 @@ struct i2_mesh {
     i2::DBuf<int32_t> d_simple, d_attached;
     i2::DBuf<int64_t> d_nbr_start, d_row_start;
     i2::DBuf<int32_t> d_nbr;
+    // disjoint (i < j, k) list, written out on request (the kernels decode it on the fly)
+    mutable i2::DBuf<int32_t> d_not_list;
+    mutable uint64_t not_list_generation = 0;
 };
 
+namespace i2 { struct RefinedMesh; }
+
+// Plain settings struct (the batched engine keeps a copy of it as its
+// configuration); the two pointers are owned by the i2_integrator_* calls only.
 struct i2_integrator {
     const i2_mesh* mesh = nullptr;
     i2d::Rule rule{};
@@ -113,6 +121,8 @@ struct i2_integrator {
     int max_level = 5;      // MAX_REFINE_LEVEL (src/common/constants.h:56)
     double tol = 1e-5;      // EPS_INTEGRATION (src/common/constants.h:18)
     int quirk = 1;          // reproduce the result-buffer swap
+    i2_evaluator* last_ev = nullptr;        // evaluator of the last run on this integrator (not owned)
+    i2::RefinedMesh* refined = nullptr;     // fixed-level refined mesh, built on request (owned)
 };
 
 namespace i2 {
@@ -128,6 +138,35 @@ struct TaskSource {
     int nT = 0;
 };
 
+// ------------------------------------------------------------ task decoding
+__device__ __forceinline__ int find_row(const int64_t* __restrict__ row_start, int nT, int64_t k) {
+    int lo = 0, hi = nT - 1;   // largest i with row_start[i] <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_start[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+// k-th disjoint (i<j) pair in lexicographic order
+__device__ __forceinline__ void decode_disjoint(const TaskSource& s, int64_t k, int* pi, int* pj) {
+    const int i = find_row(s.row_start, s.nT, k);
+    int64_t j = i + 1 + (k - s.row_start[i]);
+    for (int64_t q = s.nbr_start[i]; q < s.nbr_start[i + 1]; ++q) {
+        if (s.nbr[q] <= j) ++j; else break;
+    }
+    *pi = i; *pj = (int)j;
+}
+template <bool IMPLICIT>
+__device__ __forceinline__ void task_ij(const TaskSource& s, int64_t t, int* i, int* j) {
+    if (IMPLICIT) {
+        if (t < s.n_unordered) decode_disjoint(s, t, i, j);
+        else decode_disjoint(s, t - s.n_unordered, j, i);
+    } else {
+        *i = s.ij3[3 * t];
+        *j = s.ij3[3 * t + 1];
+    }
+}
+
 struct TypeState {
     int64_t n = 0;                 // ordered tasks
     bool implicit = false;         // disjoint all-pairs: tasks decoded on the fly
@@ -137,11 +176,16 @@ struct TypeState {
     DBuf<double> errors;           // symmetry error
     DBuf<int8_t> conv;             // convergence iteration
     DBuf<int32_t> rest[2];         // rest lists
+    DBuf<uint8_t> unconv;          // per control panel: a task still unconverged after the comparison at maxLevel
+    DBuf<uint8_t> refinements;     // per control panel: refinement passes (computed on request)
+    DBuf<int32_t> task_list;       // explicit (i, j, k) of an implicit type (materialised on request)
     int K = 0;                     // refinement iterations run
     int final_buf = 0;             // buffer holding the final integrals
     double ms = 0.0;
     int64_t refined_tasks = 0;
     bool has_errors = false;
+    bool refinements_ready = false;
+    bool task_list_ready = false;
 };
 
 }  // namespace i2
@@ -176,5 +220,12 @@ struct i2_evaluator {
 
 namespace i2 {
 int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check);
+void release_refined_mesh(i2_integrator* it);
+// Writes the mesh's disjoint (i < j, k) list to m->d_not_list (once per prepare).
+int materialize_disjoint_list(const i2_mesh* m);
+// Task source of one type of the evaluator's current task lists.
+TaskSource task_source(const i2_evaluator* ev, int type);
+// Sets flags[i] = 1 for the control panel i of each listed task (list == nullptr: tasks 0..n-1).
+int mark_unconverged(i2_evaluator* ev, bool implicit, const TaskSource& src, int64_t n, const int32_t* list, uint8_t* flags);
 int mesh_upload_and_prepare(i2_mesh* m);
 }

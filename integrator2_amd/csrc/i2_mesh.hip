@@ -127,6 +127,7 @@ int mesh_upload_and_prepare(i2_mesh* m) {
     I2_TRY(upload(m->d_nbr, m->h_nbr, m->stream));
     I2_HIP(hipStreamSynchronize(m->stream));
     m->prepared = true;
+    ++m->generation;
     return I2_OK;
 }
 
@@ -306,7 +307,7 @@ void i2_mesh_destroy(i2_mesh* m) {
         DeviceGuard g(m->device);
         (void)hipStreamSynchronize(m->stream);
         m->d_xyz.reset(); m->d_tri.reset(); m->d_normals.reset(); m->d_measures.reset(); m->d_panels.reset();
-        m->d_simple.reset(); m->d_attached.reset(); m->d_nbr_start.reset(); m->d_row_start.reset(); m->d_nbr.reset();
+        m->d_simple.reset(); m->d_attached.reset(); m->d_nbr_start.reset(); m->d_row_start.reset(); m->d_nbr.reset(); m->d_not_list.reset();
         (void)hipStreamDestroy(m->stream);
     }
     delete m;
@@ -427,6 +428,7 @@ int i2_mesh_device_array(const i2_mesh* m, int what, const void** dptr, int64_t*
         case 3: *dptr = m->d_measures.p; *n = m->nT; break;
         case 4: *dptr = m->d_simple.p; *n = m->pairs[I2_SIMPLE_NEIGHBORS]; break;
         case 5: *dptr = m->d_attached.p; *n = m->pairs[I2_ATTACHED_NEIGHBORS]; break;
+        case 6: I2_TRY(materialize_disjoint_list(m)); *dptr = m->d_not_list.p; *n = m->pairs[I2_NOT_NEIGHBORS]; break;
         default: set_error("bad array id %d", what); return I2_ERR_INVALID;
     }
     return I2_OK;

@@ -1,0 +1,294 @@
+// i2_refine.hip -- refinement diagnostics of NumericalIntegrator3D and the
+// device-side lists of Evaluator3D:
+//   * per control panel, the number of refinement passes of the last automatic
+//     run (getRefinementsRequired);
+//   * the mesh refined to the fixed level (getRefinedVertices / getRefinedCells /
+//     getRefinedCellMeasures);
+//   * the explicit (i, j, k) task list of the implicit disjoint type (getTasks);
+//   * a device-to-host copy for callers that hold only the C ABI.
+#include "i2_internal.hpp"
+#include "i2_batch.hpp"
+
+namespace i2 {
+
+struct RefinedMesh {
+    const i2_mesh* mesh = nullptr;
+    uint64_t generation = 0;
+    int level = -1;
+    int64_t nV = 0, nT = 0;
+    DBuf<double> xyz;        // double3 AoS
+    DBuf<int32_t> tri[2];    // int3 AoS, ping-pong over the levels
+    DBuf<double> meas[2];
+    int cur = 0;             // buffer holding the final cells and measures
+};
+
+void release_refined_mesh(i2_integrator* it) {
+    delete it->refined;
+    it->refined = nullptr;
+}
+
+// ------------------------------------------------------------ refinement counts
+// determineCellsToBeRefined (src/NumericalIntegrator3d.cu:461-499) adds one to
+// a cell's count every time it is called with a rest list holding a task of
+// that control panel: once before the first refinement with every task
+// (src/evaluators/evaluatorJ3DK.cu:966) and once after each comparison that
+// leaves tasks unconverged (:1008).  A task converged at iteration c is
+// therefore counted c times, one still unconverged after the comparison at
+// maxLevel maxLevel + 1 times, and a cell's count is the largest over its tasks.
+template <bool IMPLICIT>
+__global__ void k_refinement_max(int64_t n, TaskSource src, const int8_t* __restrict__ conv, int max_level,
+                                 int* __restrict__ cnt) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j;
+        task_ij<IMPLICIT>(src, t, &i, &j);
+        // maxLevel 0: the single call before the loop is the only one
+        atomicMax(&cnt[i], max_level == 0 ? 1 : (int)conv[t]);
+    }
+}
+
+__global__ void k_refinement_finish(int nT, const int* __restrict__ cnt, const uint8_t* __restrict__ unconv, int max_level,
+                                    uint8_t* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nT) out[i] = (uint8_t)((unconv && unconv[i]) ? max_level + 1 : cnt[i]);
+}
+
+template <bool IMPLICIT>
+__global__ void k_mark_unconverged(int64_t n, TaskSource src, const int32_t* __restrict__ list, uint8_t* __restrict__ flags) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        int i, j;
+        task_ij<IMPLICIT>(src, list ? (int64_t)list[q] : q, &i, &j);
+        flags[i] = 1;
+    }
+}
+
+static unsigned grid_for(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+int mark_unconverged(i2_evaluator* ev, bool implicit, const TaskSource& src, int64_t n, const int32_t* list, uint8_t* flags) {
+    if (implicit) hipLaunchKernelGGL(k_mark_unconverged<true>, dim3(grid_for(n)), dim3(256), 0, ev->stream, n, src, list, flags);
+    else hipLaunchKernelGGL(k_mark_unconverged<false>, dim3(grid_for(n)), dim3(256), 0, ev->stream, n, src, list, flags);
+    return check_launch("k_mark_unconverged");
+}
+
+static int refinement_counts(i2_evaluator* ev, int type) {
+    TypeState& ts = ev->type[type];
+    if (ts.refinements_ready) return I2_OK;
+    const i2_mesh* m = ev->mesh;
+    const int nT = (int)m->nT;
+    const int max_level = ev->integ->max_level;
+    I2_TRY(ts.refinements.ensure(nT));
+    DBuf<int> cnt;
+    I2_TRY(cnt.ensure(nT));
+    I2_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)nT, ev->stream));
+    const int8_t* conv = ts.conv.p;
+    const uint8_t* unconv = ts.unconv.p;
+    if (ev->path == 1 && ev->all_pairs) {
+        const double4* ints;
+        const double* res;
+        engine_evaluator_outputs(ev, type, &ints, &res, &conv);
+        unconv = engine_evaluator_unconverged(ev, type);
+    }
+    if (ts.n > 0) {
+        const TaskSource src = task_source(ev, type);
+        if (ts.implicit)
+            hipLaunchKernelGGL(k_refinement_max<true>, dim3(grid_for(ts.n)), dim3(256), 0, ev->stream, ts.n, src, conv, max_level, cnt.p);
+        else
+            hipLaunchKernelGGL(k_refinement_max<false>, dim3(grid_for(ts.n)), dim3(256), 0, ev->stream, ts.n, src, conv, max_level, cnt.p);
+        I2_TRY(check_launch("k_refinement_max"));
+    } else {
+        unconv = nullptr;
+    }
+    hipLaunchKernelGGL(k_refinement_finish, dim3((nT + 255) / 256), dim3(256), 0, ev->stream, nT, cnt.p, unconv, max_level,
+                       ts.refinements.p);
+    I2_TRY(check_launch("k_refinement_finish"));
+    I2_HIP(hipStreamSynchronize(ev->stream));
+    ts.refinements_ready = true;
+    return I2_OK;
+}
+
+// ------------------------------------------------------------ refined mesh
+// One refinement pass over every cell (kSplitCell, src/NumericalIntegrator3d.cu:37-87):
+// three edge midpoints 0.5 (B + C), 0.5 (C + A), 0.5 (A + B) and the children
+// {AB, B, BC}, {BC, C, CA}, {CA, A, AB}, {BC, CA, AB}, each with a quarter of the
+// parent's measure.  The reference takes the slots of cell idx from atomic
+// counters; here they are fixed at v_base + 3 idx and 4 idx, the order the
+// counters give when cells are served in index order.
+__global__ void k_split_cells(int64_t n, int64_t v_base, double* __restrict__ xyz, const int32_t* __restrict__ tri_in,
+                              const double* __restrict__ meas_in, int32_t* __restrict__ tri_out, double* __restrict__ meas_out) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t ia = tri_in[3 * c], ib = tri_in[3 * c + 1], ic = tri_in[3 * c + 2];
+        const double* A = xyz + 3 * (int64_t)ia;
+        const double* B = xyz + 3 * (int64_t)ib;
+        const double* C = xyz + 3 * (int64_t)ic;
+        const int64_t v = v_base + 3 * c;
+        double* o = xyz + 3 * v;
+        for (int d = 0; d < 3; ++d) {
+            o[d] = 0.5 * (B[d] + C[d]);
+            o[3 + d] = 0.5 * (C[d] + A[d]);
+            o[6 + d] = 0.5 * (A[d] + B[d]);
+        }
+        const int32_t bc = (int32_t)v, ca = (int32_t)(v + 1), ab = (int32_t)(v + 2);
+        int32_t* t = tri_out + 12 * c;
+        t[0] = ab; t[1] = ib; t[2] = bc;
+        t[3] = bc; t[4] = ic; t[5] = ca;
+        t[6] = ca; t[7] = ia; t[8] = ab;
+        t[9] = bc; t[10] = ca; t[11] = ab;
+        const double s = 0.25 * meas_in[c];
+        for (int k = 0; k < 4; ++k) meas_out[4 * c + k] = s;
+    }
+}
+
+static int build_refined_mesh(i2_integrator* it) {
+    const i2_mesh* m = it->mesh;
+    const int L = it->fixed_level;
+    RefinedMesh* R = it->refined;
+    if (R && R->mesh == m && R->generation == m->generation && R->level == L) return I2_OK;
+    if (!R) R = it->refined = new RefinedMesh;
+    R->mesh = nullptr;
+    R->level = -1;
+    const int64_t scale = (int64_t)1 << (2 * L);
+    const int64_t nT = m->nT * scale;
+    const int64_t nV = m->nV + m->nT * (scale - 1);
+    if (nT > (int64_t)INT32_MAX || nV > (int64_t)INT32_MAX) {
+        set_error("refined mesh of level %d has %lld cells and %lld vertices (32-bit indices)", L, (long long)nT, (long long)nV);
+        return I2_ERR_RANGE;
+    }
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_TRY(R->xyz.ensure(3 * nV));
+    I2_TRY(R->tri[0].ensure(3 * nT));
+    I2_TRY(R->meas[0].ensure(nT));
+    if (L > 0) { I2_TRY(R->tri[1].ensure(3 * nT)); I2_TRY(R->meas[1].ensure(nT)); }
+    I2_HIP(hipMemcpy(R->xyz.p, m->d_xyz.p, sizeof(double) * 3 * m->nV, hipMemcpyDeviceToDevice));
+    I2_HIP(hipMemcpy(R->tri[0].p, m->d_tri.p, sizeof(int32_t) * 3 * m->nT, hipMemcpyDeviceToDevice));
+    I2_HIP(hipMemcpy(R->meas[0].p, m->d_measures.p, sizeof(double) * m->nT, hipMemcpyDeviceToDevice));
+    int cur = 0;
+    int64_t cells = m->nT, verts = m->nV;
+    for (int l = 0; l < L; ++l) {
+        hipLaunchKernelGGL(k_split_cells, dim3(grid_for(cells)), dim3(256), 0, nullptr, cells, verts, R->xyz.p, R->tri[cur].p,
+                           R->meas[cur].p, R->tri[1 - cur].p, R->meas[1 - cur].p);
+        I2_TRY(check_launch("k_split_cells"));
+        verts += 3 * cells;
+        cells *= 4;
+        cur = 1 - cur;
+    }
+    I2_HIP(hipDeviceSynchronize());
+    R->cur = cur;
+    R->nV = nV;
+    R->nT = nT;
+    R->mesh = m;
+    R->generation = m->generation;
+    R->level = L;
+    return I2_OK;
+}
+
+// ------------------------------------------------------------ device task lists
+__global__ void k_materialize_tasks(int64_t n, TaskSource src, int32_t* __restrict__ out) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        int i, j;
+        task_ij<true>(src, t, &i, &j);
+        out[3 * t] = i;
+        out[3 * t + 1] = j;
+        out[3 * t + 2] = (int32_t)t;
+    }
+}
+
+int materialize_disjoint_list(const i2_mesh* m) {
+    if (m->not_list_generation == m->generation && (m->d_not_list.p || m->pairs[I2_NOT_NEIGHBORS] == 0)) return I2_OK;
+    const int64_t n = m->pairs[I2_NOT_NEIGHBORS];
+    if (n > (int64_t)INT32_MAX) {
+        set_error("%lld disjoint pairs: the (i, j, k) list holds 32-bit pair numbers", (long long)n);
+        return I2_ERR_RANGE;
+    }
+    DeviceGuard g(m->device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_TRY(m->d_not_list.ensure(3 * n));
+    if (n > 0) {
+        TaskSource src;
+        src.nT = (int)m->nT;
+        src.n_unordered = n;
+        src.row_start = m->d_row_start.p;
+        src.nbr_start = m->d_nbr_start.p;
+        src.nbr = m->d_nbr.p;
+        // tasks 0..n-1 of the ordered list are the (i < j) pairs themselves
+        hipLaunchKernelGGL(k_materialize_tasks, dim3(grid_for(n)), dim3(256), 0, nullptr, n, src, m->d_not_list.p);
+        I2_TRY(check_launch("k_materialize_tasks"));
+        I2_HIP(hipDeviceSynchronize());
+    }
+    m->not_list_generation = m->generation;
+    return I2_OK;
+}
+
+}  // namespace i2
+
+using namespace i2;
+
+extern "C" {
+
+int i2_integrator_refinements_required(i2_integrator* it, int type, const void** d_counts, uint8_t* h_counts, int64_t* n) {
+    if (!it || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (d_counts) *d_counts = nullptr;
+    if (n) *n = 0;
+    i2_evaluator* ev = it->last_ev;
+    if (it->fixed || !ev) return I2_OK;   // the reference allocates the counts only for automatic error control
+    if (!ev->ran) { set_error("the last run on this integrator did not complete"); return I2_ERR_STATE; }
+    DeviceGuard g(ev->mesh->device);
+    if (!g.ok) return I2_ERR_HIP;
+    I2_TRY(refinement_counts(ev, type));
+    const TypeState& ts = ev->type[type];
+    if (d_counts) *d_counts = ts.refinements.p;
+    if (n) *n = ev->mesh->nT;
+    if (h_counts) I2_HIP(hipMemcpy(h_counts, ts.refinements.p, (size_t)ev->mesh->nT, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+int i2_integrator_refined_mesh(i2_integrator* it, const void** d_vertices, const void** d_cells, const void** d_measures,
+                               int64_t* nV, int64_t* nT) {
+    if (!it) { set_error("null integrator"); return I2_ERR_INVALID; }
+    if (!it->fixed) { set_error("the refined mesh is kept for a fixed refinement level only"); return I2_ERR_STATE; }
+    if (!it->mesh->prepared) { set_error("mesh not prepared"); return I2_ERR_STATE; }
+    I2_TRY(build_refined_mesh(it));
+    const RefinedMesh* R = it->refined;
+    if (d_vertices) *d_vertices = R->xyz.p;
+    if (d_cells) *d_cells = R->tri[R->cur].p;
+    if (d_measures) *d_measures = R->meas[R->cur].p;
+    if (nV) *nV = R->nV;
+    if (nT) *nT = R->nT;
+    return I2_OK;
+}
+
+int i2_evaluator_device_tasks(i2_evaluator* ev, int type, const void** d_tasks, int64_t* n) {
+    if (!ev || !d_tasks || type < 0 || type > 2) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    TypeState& ts = ev->type[type];
+    *d_tasks = nullptr;
+    if (n) *n = ts.n;
+    if (ts.n == 0) return I2_OK;
+    if (!ts.implicit) { *d_tasks = ts.tasks.p; return I2_OK; }
+    if (ts.n > (int64_t)INT32_MAX) {
+        set_error("%lld tasks: the (i, j, k) list holds 32-bit task numbers", (long long)ts.n);
+        return I2_ERR_RANGE;
+    }
+    DeviceGuard g(ev->mesh->device);
+    if (!g.ok) return I2_ERR_HIP;
+    if (!ts.task_list_ready) {
+        I2_TRY(ts.task_list.ensure(3 * ts.n));
+        const TaskSource src = task_source(ev, type);
+        hipLaunchKernelGGL(k_materialize_tasks, dim3(grid_for(ts.n)), dim3(256), 0, ev->stream, ts.n, src, ts.task_list.p);
+        I2_TRY(check_launch("k_materialize_tasks"));
+        I2_HIP(hipStreamSynchronize(ev->stream));
+        ts.task_list_ready = true;
+    }
+    *d_tasks = ts.task_list.p;
+    return I2_OK;
+}
+
+int i2_copy_to_host(void* dst, const void* src, int64_t bytes) {
+    if (bytes < 0 || (bytes && (!dst || !src))) { set_error("invalid argument"); return I2_ERR_INVALID; }
+    if (bytes == 0) return I2_OK;
+    I2_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return I2_OK;
+}
+
+}  // extern "C"

@@ -81,11 +81,13 @@ class Oracle:
         return out
 
     def run_tasks(self, t, ij, L12, w, order, mode=1, level=5, tol=1e-5, quirk=True, K_override=-1,
-                  nthreads=0, want_levels=False, long_double=False):
-        """Integrals (n,4), J (n,3), conv (n,), K [, levels (n, level+1, 4)] for ordered tasks ij (n,2).
+                  nthreads=0, want_levels=False, long_double=False, want_unconv=False):
+        """Integrals (n,4), J (n,3), conv (n,), K [, levels (n, level+1, 4)] [, unconv (n,)] for
+        ordered tasks ij (n,2).
 
         long_double=True evaluates the level values and the singular part in
-        long double (each rounded once to double) before the Runge decisions."""
+        long double (each rounded once to double) before the Runge decisions.
+        unconv[q] = 1 where task q was unconverged after its last comparison."""
         ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
         n = len(ij)
         L12 = np.ascontiguousarray(L12, np.float64)
@@ -95,13 +97,25 @@ class Oracle:
         conv = np.zeros(n, np.int8)
         lev = np.zeros((n, level + 1, 4)) if want_levels else None
         K = ctypes.c_int()
+        unconv = np.zeros(n, np.uint8) if want_unconv else None
         lib().o_run_tasks_prec(self._h, ctypes.c_int(t), _ptr(ij), _I64(n), _ptr(L12), _ptr(w), ctypes.c_int(len(w)),
                                ctypes.c_int(order), ctypes.c_int(mode), ctypes.c_int(level), _D(tol),
                                ctypes.c_int(int(quirk)), ctypes.c_int(K_override), ctypes.c_int(nthreads),
-                               ctypes.c_int(int(long_double)), _ptr(I4), _ptr(J3), _ptr(conv), _ptr(lev), ctypes.byref(K))
-        if want_levels:
-            return I4, J3, conv, K.value, lev
-        return I4, J3, conv, K.value
+                               ctypes.c_int(int(long_double)), _ptr(I4), _ptr(J3), _ptr(conv), _ptr(lev), ctypes.byref(K),
+                               _ptr(unconv))
+        out = (I4, J3, conv, K.value) + ((lev,) if want_levels else ()) + ((unconv,) if want_unconv else ())
+        return out
+
+    def refinement_counts(self, ij, conv, unconv, max_level):
+        """Per-cell refinement counts of one type from its tasks' conv / unconv
+        (NumericalIntegrator3D::getRefinementsRequired)."""
+        ij = np.ascontiguousarray(np.asarray(ij)[:, :2], np.int32)
+        conv = np.ascontiguousarray(conv, np.int8)
+        unconv = np.ascontiguousarray(unconv, np.uint8)
+        out = np.zeros(len(self.tri), np.uint8)
+        lib().o_refinement_counts(ctypes.c_int(len(self.tri)), _ptr(ij), _I64(len(ij)), _ptr(conv), _ptr(unconv),
+                                  ctypes.c_int(max_level), _ptr(out))
+        return out
 
     def theta_psi(self, pts, tri9):
         pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)

@@ -308,7 +308,8 @@ void o_level_value(void* h, int type, int i, int j, const double* L12, const dou
 // extension of an edge line; the criterion itself stays in double.
 int o_run_tasks_prec(void* h, int type, const int32_t* ij, int64_t n, const double* L12, const double* w, int nPts,
                      int order, int mode, int level, double tol, int quirk, int K_override, int nthreads, int prec,
-                     double* out_integrals4, double* out_results3, int8_t* out_conv, double* out_levels4, int* K_out) {
+                     double* out_integrals4, double* out_results3, int8_t* out_conv, double* out_levels4, int* K_out,
+                     uint8_t* out_unconv) {
     const Mesh& m = *(Mesh*)h;
     Rule R = make_rule(L12, w, nPts, order);
     const double pow2p = (double)(1 << order);
@@ -342,13 +343,16 @@ int o_run_tasks_prec(void* h, int type, const int32_t* ij, int64_t n, const doub
         Q4<double> prev = lv(0), cur = prev;
         if (out_levels4) { double* o = out_levels4 + (size_t)q * 4 * (maxL + 1); o[0] = prev.x; o[1] = prev.y; o[2] = prev.z; o[3] = prev.w; }
         int c = 0;
+        bool still = true;   // unconverged after the last comparison made (no comparison when maxL = 0)
         for (int it = 1; it <= maxL; ++it) {
             cur = lv(it);
             if (out_levels4) { double* o = out_levels4 + (size_t)q * 4 * (maxL + 1) + 4 * it; o[0] = cur.x; o[1] = cur.y; o[2] = cur.z; o[3] = cur.w; }
             c = it;
-            if (!runge_unconverged(cur, prev, pow2p, tol)) break;
+            still = runge_unconverged(cur, prev, pow2p, tol);
+            if (!still) break;
             if (it < maxL) prev = cur;
         }
+        if (out_unconv) out_unconv[q] = still ? 1 : 0;
         if (c == 0) { finalv[q] = prev; prevv[q] = prev; }
         else { finalv[q] = cur; prevv[q] = prev; }
         conv[q] = (int8_t)c;
@@ -381,7 +385,39 @@ int o_run_tasks(void* h, int type, const int32_t* ij, int64_t n, const double* L
                 int mode, int level, double tol, int quirk, int K_override, int nthreads,
                 double* out_integrals4, double* out_results3, int8_t* out_conv, double* out_levels4, int* K_out) {
     return o_run_tasks_prec(h, type, ij, n, L12, w, nPts, order, mode, level, tol, quirk, K_override, nthreads, 0,
-                            out_integrals4, out_results3, out_conv, out_levels4, K_out);
+                            out_integrals4, out_results3, out_conv, out_levels4, K_out, nullptr);
+}
+
+// Per-cell refinement counts of one type (NumericalIntegrator3D::
+// getRefinementsRequired), by the call sequence of EvaluatorJ3DK::
+// numericalIntegration (evaluatorJ3DK.cu:962-1010): determineCellsToBeRefined
+// (NumericalIntegrator3d.cu:461-499) runs once on the full task list before the
+// loop and once after every comparison that leaves tasks in the rest list, and
+// adds one to each cell that is control panel i of a task in that list.  Task q
+// is in the rest list after the comparison at iteration `it` while it has not
+// converged: it < conv[q], or it == conv[q] with the task unconverged after its
+// last comparison (unconv[q], from o_run_tasks_prec).
+void o_refinement_counts(int nT, const int32_t* ij, int64_t n, const int8_t* conv, const uint8_t* unconv, int maxL,
+                         uint8_t* out) {
+    std::vector<int64_t> rest(n);
+    for (int64_t q = 0; q < n; ++q) rest[q] = q;
+    std::fill(out, out + nT, (uint8_t)0);
+    std::vector<uint8_t> flag(nT);
+    auto determine = [&](const std::vector<int64_t>& r) {
+        std::fill(flag.begin(), flag.end(), (uint8_t)0);
+        for (int64_t q : r) flag[ij[2 * q]] = 1;
+        for (int i = 0; i < nT; ++i) out[i] = (uint8_t)(out[i] + flag[i]);
+    };
+    if (n > 0) determine(rest);
+    int iteration = 0;
+    while (!rest.empty() && iteration < maxL) {
+        ++iteration;
+        std::vector<int64_t> next;
+        for (int64_t q : rest)
+            if (conv[q] > iteration || (conv[q] == iteration && unconv[q])) next.push_back(q);
+        rest.swap(next);
+        if (!rest.empty()) determine(rest);
+    }
 }
 
 // evaluator3d.cu:45-57 -- symmetry error for a task list laid out as

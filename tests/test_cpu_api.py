@@ -1,0 +1,232 @@
+"""The drop-in surface around the hot path, on the CPU:
+
+* the triangle rules qf3D1 ... qf3D13 of the library against the reference's
+  literals (tests/golden/quadrature_rules.json) and against the degree of
+  polynomial exactness each rule has by construction;
+* the OBJ and VTK writers (exportMeshToObj / exportMeshToVtk) against the
+  formats the reference writes: default stream formatting of doubles (C's %g,
+  six significant digits), 1-based OBJ faces, 0-based VTK connectivity,
+  offsets 3, 6, ..., one Int32 cell array per refinement-count vector;
+* the oracle's refinement-count restatement against closed forms;
+* the C++ class API header compiles with g++ against the library, its
+  quadrature constants initialise without a GPU, and the command-line driver
+  parses the reference driver's options and refuses to run without a device.
+"""
+import math
+import os
+import subprocess
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+DRIVER = os.path.join(ROOT, "integrator2_amd", "integrator2test3D")
+
+# largest total degree each rule integrates exactly (Cowper 1973)
+DEGREE = {1: 1, 3: 2, 4: 3, 6: 4, 7: 5, 9: 5, 12: 6, 13: 7}
+
+
+def golden_rules():
+    import json
+    with open(os.path.join(GOLDEN, "quadrature_rules.json")) as fh:
+        return json.load(fh)
+
+
+@pytest.mark.parametrize("name", ["qf3D1", "qf3D3", "qf3D4", "qf3D6", "qf3D7", "qf3D9", "qf3D12", "qf3D13"])
+def test_rule_table_equals_reference_literals(i2, name):
+    g = golden_rules()[name]
+    n = len(g["weights"])
+    L12, w, order = i2.quadrature_rule(n)
+    assert np.array_equal(L12, np.array([[float(a), float(b)] for a, b in g["coordinates"]]))
+    assert np.array_equal(w, np.array([float(x) for x in g["weights"]]))
+    assert order == g["order"]
+
+
+@pytest.mark.parametrize("n", sorted(DEGREE))
+def test_rule_polynomial_exactness(i2, n):
+    L12, w, _ = i2.quadrature_rule(n)
+    L1, L2 = L12[:, 0], L12[:, 1]
+    L3 = 1.0 - L1 - L2
+    d = DEGREE[n]
+    # int over a unit-area triangle of L1^a L2^b L3^c = 2 a! b! c! / (a + b + c + 2)!
+    def err(a, b, c):
+        exact = 2.0 * math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 2)
+        return abs(float(np.sum(w * L1 ** a * L2 ** b * L3 ** c)) - exact)
+    worst = max(err(a, b, c) for a in range(d + 1) for b in range(d + 1 - a) for c in range(d + 1 - a - b))
+    assert worst <= 5e-15, worst          # the literals carry 15-16 digits
+    beyond = max(err(a, b, c) for a in range(d + 2) for b in range(d + 2 - a) for c in [d + 1 - a - b])
+    assert beyond > 1e-6, beyond          # and the degree is not higher
+
+
+def test_rule_count_without_table_is_rejected(i2):
+    with pytest.raises(i2.I2Error):
+        i2.quadrature_rule(5)
+
+
+def small_mesh(i2):
+    xyz, tri = i2.icosphere(1, (1.0, 2.0, 0.5), 0.05, 3)
+    xyz[0] = [1.25e-7, -0.0, 123456.789]      # exponent and rounding cases of %g
+    return xyz, tri
+
+
+def test_obj_writer_format(i2, tmp_path):
+    xyz, tri = small_mesh(i2)
+    path = str(tmp_path / "m.obj")
+    i2.export_mesh_obj(path, xyz, tri)
+    lines = open(path).read().split("\n")
+    assert lines[-1] == ""
+    body = lines[:-1]
+    assert len(body) == len(xyz) + len(tri)
+    for v, line in zip(xyz, body[:len(xyz)]):
+        assert line == "v " + " ".join(format(x, "g") for x in v), line
+    for t, line in zip(tri, body[len(xyz):]):
+        assert line == f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}"
+
+
+def test_vtk_writer_format(i2, tmp_path):
+    xyz, tri = small_mesh(i2)
+    rng = np.random.default_rng(0)
+    ref_attached = rng.integers(0, 6, len(tri)).astype(np.uint8)
+    ref_not = rng.integers(0, 6, len(tri)).astype(np.uint8)
+    path = str(tmp_path / "m.vtp")
+    i2.export_mesh_vtk(path, xyz, tri, (None, ref_attached, ref_not))
+    text = open(path).read()
+    assert text.startswith('<?xml version="1.0" ?> \n<VTKFile type="PolyData" version="0.1" byte_order="LittleEndian">\n')
+    root = ET.fromstring(text.split("\n", 1)[1])
+    piece = root.find("PolyData/Piece")
+    assert piece.get("NumberOfPoints") == str(len(xyz)) and piece.get("NumberOfPolys") == str(len(tri))
+    pts = piece.find("Points/DataArray")
+    assert pts.get("type") == "Float32" and pts.get("NumberOfComponents") == "3"
+    assert pts.text.split() == [format(x, "g") for x in xyz.ravel()]
+    conn, offs = piece.findall("Polys/DataArray")
+    assert conn.get("Name") == "connectivity" and offs.get("Name") == "offsets"
+    assert np.array_equal(np.array(conn.text.split(), int), tri.ravel())
+    assert np.array_equal(np.array(offs.text.split(), int), 3 * np.arange(1, len(tri) + 1))
+    arrays = piece.findall("CellData/DataArray")
+    assert [a.get("Name") for a in arrays] == ["AttachedNeighborsRefinements", "NotNeighborsRefinements"]
+    assert np.array_equal(np.array(arrays[0].text.split(), int), ref_attached)
+    assert np.array_equal(np.array(arrays[1].text.split(), int), ref_not)
+    # without counts there is no CellData element
+    i2.export_mesh_vtk(path, xyz, tri)
+    root = ET.fromstring(open(path).read().split("\n", 1)[1])
+    assert root.find("PolyData/Piece/CellData") is None
+
+
+def test_writers_reject_bad_cells(i2, tmp_path):
+    xyz, tri = small_mesh(i2)
+    bad = tri.copy()
+    bad[3, 1] = len(xyz)
+    with pytest.raises(i2.I2Error):
+        i2.export_mesh_obj(str(tmp_path / "x.obj"), xyz, bad)
+    with pytest.raises(i2.I2Error):
+        i2.export_mesh_vtk(str(tmp_path / "x.vtp"), xyz, bad)
+
+
+def test_oracle_refinement_counts_closed_forms(oracle_mod, i2):
+    """The reference's count is 1 for every cell with a task after maxLevel 0;
+    after maxLevel L it is c for a cell whose tasks all converged by iteration
+    c, and L + 1 for a cell with a task unconverged at L (one call before the
+    loop and one after each comparison that leaves tasks)."""
+    xyz, tri = i2.icosphere(1, (1.0, 1.0, 1.0), 0.05, 0)
+    orc = oracle_mod.Oracle(xyz, tri)
+    nT = len(tri)
+    rng = np.random.default_rng(1)
+    ij = np.array([(i, j) for i in range(nT) for j in range(nT) if i != j and not set(tri[i]) & set(tri[j])], np.int32)
+    ij = ij[rng.permutation(len(ij))[:400]]
+    for L in (0, 1, 3):
+        conv = rng.integers(1 if L else 0, L + 1, len(ij)).astype(np.int8)
+        unconv = ((conv == L) & (rng.random(len(ij)) < 0.3)).astype(np.uint8) if L else np.ones(len(ij), np.uint8)
+        got = orc.refinement_counts(ij, conv, unconv, L)
+        for i in range(nT):
+            sel = ij[:, 0] == i
+            if not sel.any():
+                assert got[i] == 0
+            elif L == 0:
+                assert got[i] == 1
+            else:
+                expect = L + 1 if unconv[sel].any() else conv[sel].max()
+                assert got[i] == expect, (L, i, got[i], expect)
+
+
+HEADER_PROGRAM = r"""
+#include <integrator2/integrator2.hpp>
+#include <cstdio>
+// compiles the reference driver's calls (tests/integrator3D/main.cu:102-183)
+[[maybe_unused]] static void driver_shape(const std::string& f) {
+    Mesh3D mesh;
+    if (!mesh.loadMeshFromFile(f, 1.0)) return;
+    mesh.prepareMesh();
+    std::vector<Point3> vertices;
+    std::vector<int3> cells;
+    NumericalIntegrator3D numIntegrator(mesh, qf3D13);
+    EvaluatorJ3DK evaluator(mesh, numIntegrator);
+    numIntegrator.setFixedRefinementLevel(1);
+    vertices.resize(mesh.getVertices().size);
+    copy_d2h(mesh.getVertices().data, vertices.data(), mesh.getVertices().size);
+    cells.resize(mesh.getCells().size);
+    copy_d2h(mesh.getCells().data, cells.data(), mesh.getCells().size);
+    exportMeshToObj("OriginalMesh.obj", vertices, cells);
+    std::array<std::vector<unsigned char>, 3> refinements;
+    exportMeshToVtk("OriginalMesh.vtp", vertices, cells, refinements);
+    evaluator.runAllPairs(true);
+    evaluator.outputResultsToFile(neighbour_type_enum::simple_neighbors, output_format_enum::csv);
+    vertices.resize(numIntegrator.getRefinedVertices().size);
+    copy_d2h(numIntegrator.getRefinedVertices().data, vertices.data(), numIntegrator.getRefinedVertices().size);
+    for (int i = 0; i < 3; ++i) {
+        const auto* d = numIntegrator.getRefinementsRequired(neighbour_type_enum(i));
+        if (d->size) { refinements[i].resize(d->size); copy_d2h(d->data, refinements[i].data(), d->size); }
+    }
+    std::vector<int3> none;
+    evaluator.runPairs(none, none, none);
+    const deviceVector<int3>* t = evaluator.getTasks(neighbour_type_enum::not_neighbors);
+    (void)t;
+}
+int main() {
+    const QuadratureFormula3D* rules[] = {&qf3D1, &qf3D3, &qf3D4, &qf3D6, &qf3D7, &qf3D9, &qf3D12, &qf3D13};
+    for (const QuadratureFormula3D* q : rules) {
+        std::printf("%zu %d", q->weights.size(), q->order);
+        for (size_t k = 0; k < q->weights.size(); ++k)
+            std::printf(" %.17g %.17g %.17g", q->coordinates[k].x, q->coordinates[k].y, q->weights[k]);
+        std::printf("\n");
+    }
+    std::printf("%s\n", neighborTypeString(neighbour_type_enum::attached_neighbors).c_str());
+    return 0;
+}
+"""
+
+
+def test_cpp_api_header_compiles_and_rules_initialise(i2, tmp_path):
+    src = tmp_path / "api.cpp"
+    src.write_text(HEADER_PROGRAM)
+    exe = tmp_path / "api"
+    lib_dir = os.path.join(ROOT, "integrator2_amd")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-I" + os.path.join(ROOT, "include"), str(src), "-L" + lib_dir, "-lintegrator2_mi355x",
+                    "-Wl,-rpath," + lib_dir, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line, n in zip(out, [1, 3, 4, 6, 7, 9, 12, 13]):
+        f = line.split()
+        L12, w, order = i2.quadrature_rule(n)
+        assert int(f[0]) == n and int(f[1]) == order
+        vals = np.array([float(x) for x in f[2:]]).reshape(n, 3)
+        assert np.array_equal(vals[:, :2], L12) and np.array_equal(vals[:, 2], w)
+    assert out[8] == "AttachedNeighbors"
+
+
+def test_driver_options_without_device(i2, tmp_path):
+    assert os.path.exists(DRIVER), "integrator2test3D is built by integrator2_amd.build"
+    r = subprocess.run([DRIVER, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.startswith("USAGE: integrator2test3D [options]")
+    r = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert r.returncode == 0 and "USAGE" in r.stdout
+    r = subprocess.run([DRIVER, "--nosuchoption"], capture_output=True, text=True)
+    assert r.returncode == 1 and "Unknown option" in r.stderr
+    r = subprocess.run([DRIVER, "-s", "abc", "-f", "x.dat"], capture_output=True, text=True)
+    assert r.returncode == 1 and "numeric" in r.stderr
+    r = subprocess.run([DRIVER, "-c"], capture_output=True, text=True)
+    assert r.returncode == 1 and "No input file" in r.stdout
+    if i2.device_count() == 0:
+        r = subprocess.run([DRIVER, "-f", str(tmp_path / "none.dat")], capture_output=True, text=True)
+        assert r.returncode == 1 and "no HIP device" in r.stderr
