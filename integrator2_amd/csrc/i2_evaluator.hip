@@ -456,13 +456,19 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
     const i2_mesh* m = ev->mesh;
     DeviceGuard g(m->device);
     if (!g.ok) return I2_ERR_HIP;
-    I2_TRY(set_tasks(ev, I2_SIMPLE_NEIGHBORS, with_reversed(m->h_simple)));
-    I2_TRY(set_tasks(ev, I2_ATTACHED_NEIGHBORS, with_reversed(m->h_attached)));
-    TypeState& d = ev->type[I2_NOT_NEIGHBORS];
-    d.implicit = true;
-    d.task_list_ready = false;
-    d.n = 2 * m->pairs[I2_NOT_NEIGHBORS];
-    d.tasks.reset();
+    if (!ev->all_pairs_lists || ev->all_pairs_generation != m->generation) {
+        // the ordered lists depend only on the mesh: uploaded once per prepared mesh
+        ev->all_pairs_lists = false;
+        I2_TRY(set_tasks(ev, I2_SIMPLE_NEIGHBORS, with_reversed(m->h_simple)));
+        I2_TRY(set_tasks(ev, I2_ATTACHED_NEIGHBORS, with_reversed(m->h_attached)));
+        TypeState& d = ev->type[I2_NOT_NEIGHBORS];
+        d.implicit = true;
+        d.task_list_ready = false;
+        d.n = 2 * m->pairs[I2_NOT_NEIGHBORS];
+        d.tasks.reset();
+        ev->all_pairs_lists = true;
+        ev->all_pairs_generation = m->generation;
+    }
     if (ev->path == 0) return evaluator_run(ev, true, checkCorrectness != 0);
     // batched path: per-pair outputs in the same task order
     ev->ran = false;
@@ -535,6 +541,7 @@ int i2_evaluator_run_pairs(i2_evaluator* ev, const int32_t* simple3, int64_t nSi
             if (cnt != expect) { set_error("task %lld (%d, %d) shares %d vertices but was given as type %d", (long long)q, i, j, cnt, t); return I2_ERR_TOPOLOGY; }
             h[3 * q + 2] = (int32_t)q;
         }
+        ev->all_pairs_lists = false;
         I2_TRY(set_tasks(ev, t, h));
     }
     return evaluator_run(ev, false, false);

@@ -211,6 +211,9 @@ struct i2_evaluator {
     hipStream_t stream = nullptr;
     bool ran = false;
     bool all_pairs = false;
+    // the runAllPairs task lists are on the device for this mesh generation
+    bool all_pairs_lists = false;
+    uint64_t all_pairs_generation = 0;
     i2::TypeState type[3];
     i2::DBuf<unsigned long long> d_counter;
     unsigned long long* h_counter = nullptr;   // pinned

@@ -46,53 +46,188 @@ __global__ void k_panel_table(int64_t nT, const double* __restrict__ xyz, const 
     measures[c] = P.area;
 }
 
-// Neighbour classification by shared vertex indices (src/Mesh3d.cu:93-142)
-// through vertex->panel incidence, O(T * valence) instead of O(T^2): every
-// pair (i<j) sharing no vertex is disjoint.  Lists are lexicographic.
-static int classify(i2_mesh* m) {
-    const int64_t nT = m->nT, nV = m->nV;
-    std::vector<int64_t> inc_start(nV + 1, 0);
-    for (int64_t c = 0; c < nT; ++c)
-        for (int k = 0; k < 3; ++k) inc_start[m->h_tri[3 * c + k] + 1]++;
-    for (int64_t v = 0; v < nV; ++v) inc_start[v + 1] += inc_start[v];
-    std::vector<int32_t> inc(inc_start[nV]);
-    std::vector<int64_t> fill(inc_start.begin(), inc_start.end() - 1);
-    for (int64_t c = 0; c < nT; ++c)
-        for (int k = 0; k < 3; ++k) inc[fill[m->h_tri[3 * c + k]]++] = (int32_t)c;
-    m->h_simple.clear(); m->h_attached.clear();
-    m->h_nbr_start.assign(nT + 1, 0);
-    m->h_nbr.clear();
-    m->h_row_start.assign(nT + 1, 0);
-    std::vector<int32_t> cand;
-    for (int64_t i = 0; i < nT; ++i) {
-        cand.clear();
-        for (int k = 0; k < 3; ++k) {
-            const int32_t v = m->h_tri[3 * i + k];
-            for (int64_t q = inc_start[v]; q < inc_start[v + 1]; ++q)
-                if (inc[q] > i) cand.push_back(inc[q]);
-        }
-        std::sort(cand.begin(), cand.end());
-        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-        int64_t excluded = 0;
-        for (int32_t j : cand) {
-            int cnt = 0;
-            for (int a = 0; a < 3; ++a) {
-                const int32_t va = m->h_tri[3 * i + a];
-                if (va == m->h_tri[3 * j] || va == m->h_tri[3 * j + 1] || va == m->h_tri[3 * j + 2]) ++cnt;
-            }
-            if (cnt == 1) { m->h_simple.push_back((int32_t)i); m->h_simple.push_back(j); m->h_simple.push_back(0); }
-            else if (cnt == 2) { m->h_attached.push_back((int32_t)i); m->h_attached.push_back(j); m->h_attached.push_back(0); }
-            // every candidate shares a vertex, so it is excluded from the disjoint row
-            m->h_nbr.push_back(j);
-            ++excluded;
-        }
-        m->h_nbr_start[i + 1] = (int64_t)m->h_nbr.size();
-        m->h_row_start[i + 1] = m->h_row_start[i] + (nT - 1 - i) - excluded;
+// ------------------------------------------------------------ neighbour classification
+// kDetermineNeighborType (src/Mesh3d.cu:93-142) counts, for every pair i < j,
+// how many of cell i's three vertex slots occur in cell j: 1 -> vertex-sharing,
+// 2 -> edge-sharing, 0 -> disjoint (3, a repeated cell, joins no list).  Here
+// the count comes from vertex -> cell incidence in O(T * valence): the sorted
+// cell lists of i's three slot vertices are merged, and a cell j > i occurring
+// in m of them shares m slots.  Every merged j is a neighbour, so the disjoint
+// pairs of row i are all j > i except those; they are never listed, only
+// counted (row_start) and decoded on the fly from the sorted neighbour list.
+// The vertex- and edge-sharing lists come out in lexicographic (i, j) order.
+
+// Exclusive prefix sum of n counts into out[0..n] (out[n] = total), one
+// workgroup: each thread scans a contiguous chunk.
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_counts(int64_t n, const T* __restrict__ cnt, int64_t* __restrict__ out) {
+    __shared__ int64_t part[1024];
+    const int64_t chunk = (n + blockDim.x - 1) / blockDim.x;
+    const int64_t b = threadIdx.x * chunk, e = b + chunk < n ? b + chunk : n;
+    int64_t s = 0;
+    for (int64_t q = b; q < e; ++q) s += (int64_t)cnt[q];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (unsigned t = 0; t < blockDim.x; ++t) { const int64_t v = part[t]; part[t] = run; run += v; }
+        out[n] = run;
     }
-    for (size_t q = 0; q < m->h_simple.size() / 3; ++q) m->h_simple[3 * q + 2] = (int32_t)q;
-    for (size_t q = 0; q < m->h_attached.size() / 3; ++q) m->h_attached[3 * q + 2] = (int32_t)q;
-    m->pairs[I2_SIMPLE_NEIGHBORS] = (int64_t)m->h_simple.size() / 3;
-    m->pairs[I2_ATTACHED_NEIGHBORS] = (int64_t)m->h_attached.size() / 3;
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t q = b; q < e; ++q) { out[q] = run; run += (int64_t)cnt[q]; }
+}
+
+__global__ void k_vertex_degree(int64_t nslots, const int32_t* __restrict__ tri, int32_t* __restrict__ deg) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&deg[tri[s]], 1);
+}
+
+__global__ void k_fill_incidence(int64_t nslots, const int32_t* __restrict__ tri, const int64_t* __restrict__ start,
+                                 int32_t* __restrict__ cursor, int32_t* __restrict__ inc) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = tri[s];
+        inc[start[v] + atomicAdd(&cursor[v], 1)] = (int32_t)(s / 3);
+    }
+}
+
+// Per vertex: its cell list sorted ascending (insertion sort; lists are short).
+__global__ void k_sort_incidence(int64_t nV, const int64_t* __restrict__ start, int32_t* __restrict__ inc) {
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nV; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = start[v], e = start[v + 1];
+        for (int64_t q = b + 1; q < e; ++q) {
+            const int32_t x = inc[q];
+            int64_t r = q - 1;
+            while (r >= b && inc[r] > x) { inc[r + 1] = inc[r]; --r; }
+            inc[r + 1] = x;
+        }
+    }
+}
+
+struct SlotList {
+    const int32_t* p;
+    int64_t q, e;
+    // first entry > i
+    __device__ void seek(int32_t i) {
+        int64_t lo = q, hi = e;
+        while (lo < hi) { const int64_t mid = (lo + hi) >> 1; if (p[mid] <= i) lo = mid + 1; else hi = mid; }
+        q = lo;
+    }
+    __device__ int32_t head() const { return q < e ? p[q] : INT32_MAX; }
+    // advances past every copy of j (a cell with a repeated vertex is listed twice)
+    __device__ int take(int32_t j) {
+        if (q >= e || p[q] != j) return 0;
+        while (q < e && p[q] == j) ++q;
+        return 1;
+    }
+};
+
+// WRITE = false: per cell i, the numbers of vertex-sharing, edge-sharing and
+// all sharing cells j > i.  WRITE = true: the lists at the scanned offsets.
+template <bool WRITE>
+__global__ void k_classify(int64_t nT, const int32_t* __restrict__ tri, const int64_t* __restrict__ start,
+                           const int32_t* __restrict__ inc, int32_t* __restrict__ n_simple, int32_t* __restrict__ n_attached,
+                           int32_t* __restrict__ n_shared, const int64_t* __restrict__ off_simple,
+                           const int64_t* __restrict__ off_attached, const int64_t* __restrict__ off_shared,
+                           int32_t* __restrict__ simple, int32_t* __restrict__ attached, int32_t* __restrict__ nbr) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nT; c += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t i = (int32_t)c;
+        SlotList L[3];
+        for (int k = 0; k < 3; ++k) {
+            const int32_t v = tri[3 * c + k];
+            L[k].p = inc; L[k].q = start[v]; L[k].e = start[v + 1];
+            L[k].seek(i);
+        }
+        int64_t ws = WRITE ? off_simple[c] : 0, wa = WRITE ? off_attached[c] : 0, wn = WRITE ? off_shared[c] : 0;
+        int32_t cs = 0, ca = 0, cn = 0;
+        for (;;) {
+            const int32_t j = min(L[0].head(), min(L[1].head(), L[2].head()));
+            if (j == INT32_MAX) break;
+            const int m = L[0].take(j) + L[1].take(j) + L[2].take(j);
+            ++cn;
+            if (m == 1) ++cs;
+            if (m == 2) ++ca;
+            if (WRITE) {
+                nbr[wn++] = j;
+                if (m == 1) { simple[3 * ws] = i; simple[3 * ws + 1] = j; simple[3 * ws + 2] = (int32_t)ws; ++ws; }
+                if (m == 2) { attached[3 * wa] = i; attached[3 * wa + 1] = j; attached[3 * wa + 2] = (int32_t)wa; ++wa; }
+            }
+        }
+        if (!WRITE) { n_simple[c] = cs; n_attached[c] = ca; n_shared[c] = cn; }
+    }
+}
+
+// Disjoint pairs of row i: the nT - 1 - i cells j > i less those sharing a vertex.
+__global__ void k_disjoint_row_counts(int64_t nT, const int32_t* __restrict__ n_shared, int64_t* __restrict__ row) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nT; c += (int64_t)gridDim.x * blockDim.x)
+        row[c] = (nT - 1 - c) - (int64_t)n_shared[c];
+}
+
+static unsigned grid_of(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 65536 ? 65536 : b));
+}
+
+template <class T>
+static int download(std::vector<T>& h, const DBuf<T>& d, int64_t n, hipStream_t s) {
+    h.resize((size_t)n);
+    if (n) I2_HIP(hipMemcpyAsync(h.data(), d.p, sizeof(T) * (size_t)n, hipMemcpyDeviceToHost, s));
+    return I2_OK;
+}
+
+static int classify(i2_mesh* m) {
+    const int64_t nT = m->nT, nV = m->nV, nslots = 3 * nT;
+    hipStream_t st = m->stream;
+    DBuf<int32_t> deg, cursor, inc, ns, na, nn;
+    DBuf<int64_t> start, off_s, off_a, rows;
+    I2_TRY(deg.ensure(nV)); I2_TRY(cursor.ensure(nV)); I2_TRY(inc.ensure(nslots));
+    I2_TRY(start.ensure(nV + 1));
+    I2_TRY(ns.ensure(nT)); I2_TRY(na.ensure(nT)); I2_TRY(nn.ensure(nT));
+    I2_TRY(off_s.ensure(nT + 1)); I2_TRY(off_a.ensure(nT + 1)); I2_TRY(rows.ensure(nT));
+    I2_TRY(m->d_nbr_start.ensure(nT + 1)); I2_TRY(m->d_row_start.ensure(nT + 1));
+    I2_HIP(hipMemsetAsync(deg.p, 0, sizeof(int32_t) * nV, st));
+    I2_HIP(hipMemsetAsync(cursor.p, 0, sizeof(int32_t) * nV, st));
+    hipLaunchKernelGGL(k_vertex_degree, dim3(grid_of(nslots)), dim3(256), 0, st, nslots, m->d_tri.p, deg.p);
+    I2_TRY(check_launch("k_vertex_degree"));
+    hipLaunchKernelGGL(k_scan_counts<int32_t>, dim3(1), dim3(1024), 0, st, nV, deg.p, start.p);
+    I2_TRY(check_launch("k_scan_counts"));
+    hipLaunchKernelGGL(k_fill_incidence, dim3(grid_of(nslots)), dim3(256), 0, st, nslots, m->d_tri.p, start.p, cursor.p, inc.p);
+    I2_TRY(check_launch("k_fill_incidence"));
+    hipLaunchKernelGGL(k_sort_incidence, dim3(grid_of(nV)), dim3(256), 0, st, nV, start.p, inc.p);
+    I2_TRY(check_launch("k_sort_incidence"));
+    hipLaunchKernelGGL(k_classify<false>, dim3(grid_of(nT)), dim3(256), 0, st, nT, m->d_tri.p, start.p, inc.p, ns.p, na.p, nn.p,
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    I2_TRY(check_launch("k_classify"));
+    hipLaunchKernelGGL(k_scan_counts<int32_t>, dim3(1), dim3(1024), 0, st, nT, ns.p, off_s.p);
+    hipLaunchKernelGGL(k_scan_counts<int32_t>, dim3(1), dim3(1024), 0, st, nT, na.p, off_a.p);
+    hipLaunchKernelGGL(k_scan_counts<int32_t>, dim3(1), dim3(1024), 0, st, nT, nn.p, m->d_nbr_start.p);
+    hipLaunchKernelGGL(k_disjoint_row_counts, dim3(grid_of(nT)), dim3(256), 0, st, nT, nn.p, rows.p);
+    hipLaunchKernelGGL(k_scan_counts<int64_t>, dim3(1), dim3(1024), 0, st, nT, rows.p, m->d_row_start.p);
+    I2_TRY(check_launch("k_scan_counts"));
+    int64_t tot[3];
+    I2_HIP(hipMemcpyAsync(&tot[0], off_s.p + nT, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    I2_HIP(hipMemcpyAsync(&tot[1], off_a.p + nT, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    I2_HIP(hipMemcpyAsync(&tot[2], m->d_nbr_start.p + nT, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    I2_HIP(hipStreamSynchronize(st));
+    if (tot[0] > (int64_t)INT32_MAX || tot[1] > (int64_t)INT32_MAX) {
+        set_error("%lld vertex-sharing / %lld edge-sharing pairs exceed the 32-bit pair numbers", (long long)tot[0], (long long)tot[1]);
+        return I2_ERR_RANGE;
+    }
+    I2_TRY(m->d_simple.ensure(3 * tot[0]));
+    I2_TRY(m->d_attached.ensure(3 * tot[1]));
+    I2_TRY(m->d_nbr.ensure(tot[2]));
+    hipLaunchKernelGGL(k_classify<true>, dim3(grid_of(nT)), dim3(256), 0, st, nT, m->d_tri.p, start.p, inc.p, nullptr, nullptr, nullptr,
+                       off_s.p, off_a.p, m->d_nbr_start.p, m->d_simple.p, m->d_attached.p, m->d_nbr.p);
+    I2_TRY(check_launch("k_classify"));
+    // host copies for the host-side task lists and getters
+    I2_TRY(download(m->h_simple, m->d_simple, 3 * tot[0], st));
+    I2_TRY(download(m->h_attached, m->d_attached, 3 * tot[1], st));
+    I2_TRY(download(m->h_nbr_start, m->d_nbr_start, nT + 1, st));
+    I2_TRY(download(m->h_row_start, m->d_row_start, nT + 1, st));
+    I2_TRY(download(m->h_nbr, m->d_nbr, tot[2], st));
+    I2_HIP(hipStreamSynchronize(st));
+    m->pairs[I2_SIMPLE_NEIGHBORS] = tot[0];
+    m->pairs[I2_ATTACHED_NEIGHBORS] = tot[1];
     m->pairs[I2_NOT_NEIGHBORS] = m->h_row_start[nT];
     return I2_OK;
 }
@@ -109,9 +244,9 @@ int mesh_upload_and_prepare(i2_mesh* m) {
     if (!g.ok) return I2_ERR_HIP;
     if (m->nT <= 0 || m->nV <= 0) { set_error("mesh is empty"); return I2_ERR_STATE; }
     if (m->nT > (int64_t)INT32_MAX) { set_error("more than 2^31-1 cells"); return I2_ERR_RANGE; }
-    I2_TRY(classify(m));
     I2_TRY(upload(m->d_xyz, m->h_xyz, m->stream));
     I2_TRY(upload(m->d_tri, m->h_tri, m->stream));
+    I2_TRY(classify(m));
     I2_TRY(m->d_normals.ensure(3 * m->nT));
     I2_TRY(m->d_measures.ensure(m->nT));
     I2_TRY(m->d_panels.ensure(m->nT));
@@ -120,11 +255,6 @@ int mesh_upload_and_prepare(i2_mesh* m) {
     hipLaunchKernelGGL(k_panel_table, dim3((unsigned)blocks), dim3(threads), 0, m->stream, m->nT, m->d_xyz.p, m->d_tri.p,
                        m->d_panels.p, m->d_normals.p, m->d_measures.p);
     I2_TRY(check_launch("k_panel_table"));
-    I2_TRY(upload(m->d_simple, m->h_simple, m->stream));
-    I2_TRY(upload(m->d_attached, m->h_attached, m->stream));
-    I2_TRY(upload(m->d_nbr_start, m->h_nbr_start, m->stream));
-    I2_TRY(upload(m->d_row_start, m->h_row_start, m->stream));
-    I2_TRY(upload(m->d_nbr, m->h_nbr, m->stream));
     I2_HIP(hipStreamSynchronize(m->stream));
     m->prepared = true;
     ++m->generation;

@@ -192,3 +192,45 @@ def test_driver_end_to_end(gpu, tmp_path):
     assert piece.get("NumberOfPoints") == str(V + 3 * T) and piece.find("CellData") is None
     assert os.path.exists(run_fixed / "OriginalMesh.vtp")
     assert all(os.path.exists(run_fixed / f"{n}.dat") for n in names)
+
+
+def brute_force_lists(tri):
+    """kDetermineNeighborType's definition (src/Mesh3d.cu:93-142): for i < j,
+    the number of cell i's three vertex slots that occur in cell j."""
+    T = len(tri)
+    simple, attached, disjoint = [], [], []
+    for i in range(T):
+        for j in range(i + 1, T):
+            m = sum(int(tri[i][k] in tri[j]) for k in range(3))
+            (disjoint if m == 0 else simple if m == 1 else attached if m == 2 else []).append((i, j))
+    return simple, attached, disjoint
+
+
+def test_classification_on_device(gpu):
+    """The incidence-merge classification equals the brute-force definition,
+    including repeated cells, cells with a repeated vertex and a vertex of
+    valence 40; the counts at full size equal the icosphere's closed forms."""
+    rng = np.random.default_rng(7)
+    xyz, tri = gpu.icosphere(1, (1.0, 1.0, 1.0), 0.0, 0)
+    V = len(xyz)
+    fan = np.array([[V, 1 + k % (V - 1), 1 + (k + 1) % (V - 1)] for k in range(40)], np.int32)
+    extra = np.array([tri[5], [tri[7][0], tri[7][0], tri[7][1]], [3, 3, 3]], np.int32)
+    tri2 = np.concatenate([tri, fan, extra])[rng.permutation(len(tri) + 43)]
+    xyz2 = np.concatenate([xyz, [[0.0, 0.0, 2.0]]])
+    m = gpu.Mesh3D()
+    m.setData(xyz2, tri2)
+    m.prepareMesh()
+    s, a, d = brute_force_lists([list(t) for t in tri2])
+    k = lambda l: np.array([(i, j, q) for q, (i, j) in enumerate(l)], np.int32).reshape(-1, 3)
+    assert np.array_equal(m.getSimpleNeighbors(), k(s))
+    assert np.array_equal(m.getAttachedNeighbors(), k(a))
+    assert np.array_equal(m.getNotNeighbors(), k(d))
+    assert m.sizes()[2] == [len(s), len(a), len(d)]
+    for lev in (4, 5):
+        xyz, tri = gpu.icosphere(lev)
+        m = gpu.Mesh3D()
+        m.setData(xyz, tri)
+        m.prepareMesh()
+        V, T = len(xyz), len(tri)
+        vertex, edge = 60 + 9 * (V - 12), 3 * T // 2
+        assert m.sizes()[2] == [vertex, edge, T * (T - 1) // 2 - vertex - edge], lev
