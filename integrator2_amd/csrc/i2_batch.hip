@@ -24,8 +24,10 @@
 //              min(4^L, 64) lanes per task, one sub-panel per lane,
 //              fixed-order butterfly reduction.
 //   k_levels / k_compute_K   K per (mesh, type) from queue membership.
-//   k_finalize_disjoint (per row of the disjoint pair list), k_finalize_singular,
-//   k_finalize_rows   parity selection, J, outputs.
+//   per-pair outputs (value, J) are written by the task's last kernel for
+//   K = maxLevel; k_finalize_disjoint (per row of the disjoint pair list) and
+//   k_finalize_singular correct the types whose K differs by an odd number;
+//   k_finalize_rows   row sums for K's parity.
 #include "i2_batch.hpp"
 #include "i2_adjoint.hpp"
 
@@ -197,8 +199,13 @@ __device__ __forceinline__ void push_deep(const BatchArgs& A, int level, int typ
 
 // Writes a finished task: per-pair arrays, or its contribution to the row sums
 // of both parities of K (returned in rp0/rp1 as (J, Theta)).
+// Per-pair outputs are written for K = maxLevel (the fixed level in fixed
+// mode), which is K whenever any task of the type reaches maxLevel: fin holds
+// the value the buffer swap reports for that K, prev the other one, res its J.
+// Where K turns out smaller by an odd number, the finalize kernels swap the two
+// and recompute J.
 template <int TYPE, int OUT>
-__device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& M, int64_t z, int i, const StepOut& o,
+__device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& M, int64_t z, int i, int j, const StepOut& o,
                                             const Panel& I, const Panel& J, q4* rp0, q4* rp1) {
     q4 cur = o.cur, prev = o.prev;
     if (TYPE != I2_NOT_NEIGHBORS) {
@@ -207,10 +214,18 @@ __device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& 
         prev = prev + s;
     }
     if (OUT == OUT_PER_PAIR) {
-        M.fin[TYPE][z] = make_double4(cur.x, cur.y, cur.z, cur.w);
-        M.prev[TYPE][z] = make_double4(prev.x, prev.y, prev.z, prev.w);
+        const bool swap = A.quirk && o.c >= 1 && ((A.last_level - o.c) & 1);
+        const q4 rep = swap ? prev : cur, other = swap ? cur : prev;
+        M.fin[TYPE][z] = make_double4(rep.x, rep.y, rep.z, rep.w);
+        M.prev[TYPE][z] = make_double4(other.x, other.y, other.z, other.w);
         M.conv[TYPE][z] = (int8_t)o.c;
         if (o.unconv && M.unconv[TYPE]) M.unconv[TYPE][i] = 1;
+        // the panel records are re-read here so that no panel data stays live
+        // across the quadrature loop
+        const d3 nJ = M.panels[j].n;
+        const d3 r = (TYPE == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(rep, nJ, M.panels[i].area) : finalize_plain(rep, nJ);
+        double* out = M.res[TYPE] + 3 * z;
+        out[0] = r.x; out[1] = r.y; out[2] = r.z;
     } else {
         if (M.conv[TYPE]) M.conv[TYPE][z] = (int8_t)o.c;   // kept for the adjoint
         // K even selects I_c for even c, I_{c-1} for odd c; K odd the opposite
@@ -312,7 +327,7 @@ __global__ __launch_bounds__(256) I2_THREE_WAVES void k_sweep_disjoint(BatchArgs
         const int64_t z = valid ? disjoint_task(M, i, j) : 0;
         const StepOut o = sweep_task<I2_NOT_NEIGHBORS, LPP>(A, R, m, i, j, z, valid, s, I, J, &pts);
         q4 rp0 = q4zero(), rp1 = q4zero();
-        if (o.finished) finish_task<I2_NOT_NEIGHBORS, OUT>(A, M, z, i, o, I, J, &rp0, &rp1);
+        if (o.finished) finish_task<I2_NOT_NEIGHBORS, OUT>(A, M, z, i, j, o, I, J, &rp0, &rp1);
         if (OUT == OUT_ROW_SUMS) {
             // i is uniform: reduce over the wavefront (or the whole smaller
             // workgroup), one set of atomics per wavefront
@@ -358,13 +373,13 @@ __global__ __launch_bounds__(256) I2_TWO_WAVES void k_sweep_singular(BatchArgs A
         if (type == I2_SIMPLE_NEIGHBORS) {
             const StepOut o = sweep_task<I2_SIMPLE_NEIGHBORS, 1>(A, R, m, i, j, z, valid, 0, I, J);
             if (o.finished) {
-                finish_task<I2_SIMPLE_NEIGHBORS, OUT>(A, M, z, i, o, I, J, &rp0, &rp1);
+                finish_task<I2_SIMPLE_NEIGHBORS, OUT>(A, M, z, i, j, o, I, J, &rp0, &rp1);
                 if (OUT == OUT_ROW_SUMS) add_row(M, type, i, rp0, rp1);
             }
         } else {
             const StepOut o = sweep_task<I2_ATTACHED_NEIGHBORS, 1>(A, R, m, i, j, z, valid, 0, I, J);
             if (o.finished) {
-                finish_task<I2_ATTACHED_NEIGHBORS, OUT>(A, M, z, i, o, I, J, &rp0, &rp1);
+                finish_task<I2_ATTACHED_NEIGHBORS, OUT>(A, M, z, i, j, o, I, J, &rp0, &rp1);
                 if (OUT == OUT_ROW_SUMS) add_row(M, type, i, rp0, rp1);
             }
         }
@@ -404,7 +419,7 @@ __device__ __forceinline__ void deep_task(const BatchArgs& A, const RuleLDS& R, 
     }
     if (o.finished) {
         q4 rp0, rp1;
-        finish_task<TYPE, OUT>(A, M, e.z, e.i, o, I, J, &rp0, &rp1);
+        finish_task<TYPE, OUT>(A, M, e.z, e.i, e.j, o, I, J, &rp0, &rp1);
         if (OUT == OUT_ROW_SUMS) add_row(M, TYPE, e.i, rp0, rp1);
     }
 }
@@ -474,18 +489,21 @@ __global__ void k_compute_K(BatchArgs A) {
     A.K[g] = K;
 }
 
-// Parity selection and J for per-pair outputs, grid-stride over all tasks.
-// The reported value of ordered task z of a type (I_c, or I_(c-1) where the
-// buffer swap selects it) and its J.
+// Per-pair outputs were written for K = maxLevel.  Where the type's K differs
+// from it by an odd number (the buffer swap reproduced), every task with
+// c >= 1 reports the other level: fin and prev trade places and J is
+// recomputed.  Otherwise nothing is left to do.
+__device__ __forceinline__ bool needs_correction(const BatchArgs& A, int m, int type) {
+    return A.quirk && ((A.K[3 * m + type] - A.last_level) & 1);
+}
 __device__ __forceinline__ void finalize_task(const BatchArgs& A, const MeshDesc& M, int m, int type, int64_t z, int i, int j) {
-    const int K = A.K[3 * m + type];
-    const int c = M.conv[type][z];
-    const bool swap = A.quirk && c >= 1 && ((K - c) & 1);
-    const double4 v4 = swap ? M.prev[type][z] : M.fin[type][z];
-    const q4 v = {v4.x, v4.y, v4.z, v4.w};
+    if (M.conv[type][z] < 1) return;
+    const double4 a = M.fin[type][z], b = M.prev[type][z];
+    M.fin[type][z] = b;
+    M.prev[type][z] = a;
+    const q4 v = {b.x, b.y, b.z, b.w};
     const d3 nJ = M.panels[j].n;
     const d3 r = (type == I2_SIMPLE_NEIGHBORS) ? finalize_vertex(v, nJ, M.panels[i].area) : finalize_plain(v, nJ);
-    if (swap) M.fin[type][z] = v4;   // the reported value replaces I_c only where the swap selects I_(c-1)
     double* o = M.res[type] + 3 * z;
     o[0] = r.x; o[1] = r.y; o[2] = r.z;
 }
@@ -496,6 +514,7 @@ __device__ __forceinline__ void finalize_task(const BatchArgs& A, const MeshDesc
 __global__ __launch_bounds__(256) void k_finalize_disjoint(BatchArgs A) {
     for (int64_t r = blockIdx.x; r < A.total_rows; r += gridDim.x) {
         const int m = find_base(A.row_base, A.nmeshes, r);
+        if (!needs_correction(A, m, I2_NOT_NEIGHBORS)) continue;
         const MeshDesc& M = A.meshes[m];
         const int ii = (int)(r - A.row_base[m]);
         const int64_t k0 = M.row_start[ii], cnt = M.row_start[ii + 1] - k0;
@@ -520,6 +539,7 @@ __global__ __launch_bounds__(256) void k_finalize_singular(BatchArgs A) {
         const int b = find_base(A.sing_base, 2 * A.nmeshes, item);
         const int type = b / A.nmeshes == 0 ? I2_SIMPLE_NEIGHBORS : I2_ATTACHED_NEIGHBORS;
         const int m = b % A.nmeshes;
+        if (!needs_correction(A, m, type)) continue;
         const MeshDesc& M = A.meshes[m];
         const int64_t n = M.npairs[type];
         const int64_t z = (item - A.sing_base[b]) * 256 + (g & 255);

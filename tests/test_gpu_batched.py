@@ -21,6 +21,19 @@ def test_batched_auto_vs_oracle(gpu, oracle_mod, quirk):
     check_auto(gpu, oracle_mod, xyz, tri, 3, 1e-8, quirk, path=1)
 
 
+@pytest.mark.parametrize("extra", [1, 2])
+def test_batched_auto_with_K_below_max_level(gpu, oracle_mod, extra):
+    """The batched kernels write per-pair outputs for K = maxLevel and correct
+    them afterwards where K ends below it by an odd number: maxLevel = K + 1
+    (every task's reported level changes parity) and K + 2 (none does)."""
+    xyz, tri = gpu.icosphere(2, (2.0, 1.0, 0.7), 0.05, 0)
+    _, _, Ks = run_gpu(gpu, xyz, tri, maxLevel=6, tol=1e-8, path=1, check=False)
+    maxL = max(Ks.values()) + extra
+    print(f"K per type {Ks}, maxLevel {maxL}", flush=True)
+    assert max(Ks.values()) < 6
+    check_auto(gpu, oracle_mod, xyz, tri, maxL, 1e-8, True, path=1)
+
+
 @pytest.mark.parametrize("level", [1, 2])
 def test_batched_fixed_levels_vs_oracle(gpu, oracle_mod, level):
     xyz, tri = gpu.icosphere(1, (2.0, 1.0, 0.7), 0.05, 5)
