@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <fstream>
+#include <mutex>
+#include <unordered_set>
 
 namespace i2 {
 using namespace i2d;
@@ -358,6 +360,10 @@ static int set_tasks(i2_evaluator* ev, int t, const std::vector<int32_t>& h) {
 
 using namespace i2;
 
+// Live integrators: an evaluator destroyed after its integrator must not touch it.
+static std::mutex g_integrators_mutex;
+static std::unordered_set<const i2_integrator*> g_integrators;
+
 extern "C" {
 
 int i2_integrator_create(i2_integrator** out, const i2_mesh* mesh, const double* L12, const double* w, int nPts, int order) {
@@ -379,11 +385,19 @@ int i2_integrator_create(i2_integrator** out, const i2_mesh* mesh, const double*
             it->rule.L1[k] = it->rule.L2[k] = it->rule.L3[k] = it->rule.w[k] = 0.0;
         }
     }
+    {
+        std::lock_guard<std::mutex> lock(g_integrators_mutex);
+        g_integrators.insert(it);
+    }
     *out = it;
     return I2_OK;
 }
 void i2_integrator_destroy(i2_integrator* it) {
     if (!it) return;
+    {
+        std::lock_guard<std::mutex> lock(g_integrators_mutex);
+        g_integrators.erase(it);
+    }
     release_refined_mesh(it);
     delete it;
 }
@@ -438,7 +452,10 @@ void i2_evaluator_destroy(i2_evaluator* ev) {
             ts.conv.reset(); ts.rest[0].reset(); ts.rest[1].reset();
             ts.unconv.reset(); ts.refinements.reset(); ts.task_list.reset();
         }
-        if (ev->integ && ev->integ->last_ev == ev) ev->integ->last_ev = nullptr;
+        {
+            std::lock_guard<std::mutex> lock(g_integrators_mutex);
+            if (g_integrators.count(ev->integ) && ev->integ->last_ev == ev) ev->integ->last_ev = nullptr;
+        }
         clear_launches(ev);
         engine_release_evaluator(ev);
         adjoint_release(ev->adj);
