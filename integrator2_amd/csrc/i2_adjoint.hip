@@ -350,6 +350,9 @@ __device__ __forceinline__ D3 dual_vertex(d3 x, int slot, int my_slot, int axis)
     return r;
 }
 __device__ __forceinline__ double dd(d3 bar, D3 x) { return bar.x * x.x.d + bar.y * x.y.d + bar.z * x.z.d; }
+// a, b or c by a run-time index, as selects (an indexed local array would live in scratch memory)
+template <class T>
+__device__ __forceinline__ T pick3(int s, const T& a, const T& b, const T& c) { return s == 0 ? a : (s == 1 ? b : c); }
 
 struct SingBars {
     TPBar g;          // thetaPsi parameters (K_j vertices, tangents)
@@ -441,26 +444,27 @@ __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R,
     const D3 nI = unit(cri), nJ = unit(crj);
     const Dual SI = vlen(cri) * Dual(0.5);
     const D3 ta = unit(jc - jb), tb = unit(ja - jc), tc = unit(jb - ja);
-    const D3 vi[3] = {ia, ib, ic}, vj[3] = {ja, jb, jc};
+    const D3 i0 = pick3(s1, ia, ib, ic), i1 = pick3((s1 + 1) % 3, ia, ib, ic), i2v = pick3((s1 + 2) % 3, ia, ib, ic);
+    const D3 j0 = pick3(s2, ja, jb, jc), j1 = pick3((s2 + 1) % 3, ja, jb, jc), j2 = pick3((s2 + 2) % 3, ja, jb, jc);
     double dl = 0.0;
-    if (slot < 3) dl += (axis == 0 ? B.X[slot].x : (axis == 1 ? B.X[slot].y : B.X[slot].z));
-    else {
+    if (slot < 3) {
+        const d3 gx = pick3(slot, B.X[0], B.X[1], B.X[2]);
+        dl += (axis == 0 ? gx.x : (axis == 1 ? gx.y : gx.z));
+    } else {
         const d3 gd = slot == 3 ? B.g.a : (slot == 4 ? B.g.b : B.g.c);
         dl += (axis == 0 ? gd.x : (axis == 1 ? gd.y : gd.z));
     }
     dl += dd(B.g.ta, ta) + dd(B.g.tb, tb) + dd(B.g.tc, tc) + dd(nb, nJ) + Sb * SI.d;
     Q4 sd;
     if (TYPE == I2_ATTACHED_NEIGHBORS) {
-        const EdgeCtxT<Dual> c = edge_ctx(vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3]);
+        const EdgeCtxT<Dual> c = edge_ctx(j0, j1, j2);
         dl += dd(B.ge.JB, c.JB) + dd(B.ge.JC, c.JC) + dd(B.ge.taua, c.taua) + dd(B.ge.taub, c.taub) +
               dd(B.ge.tauc, c.tauc) + B.ge.ilvc * c.ilvc.d;
-        sd = integrate_sing_edge(vi[s1], vi[(s1 + 1) % 3], vi[(s1 + 2) % 3], vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3],
-                                 nI, nJ, SI);
+        sd = integrate_sing_edge(i0, i1, i2v, j0, j1, j2, nI, nJ, SI);
     } else {
-        const VertexCtxT<Dual> c = vertex_ctx(vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3], nI, nJ, SI);
+        const VertexCtxT<Dual> c = vertex_ctx(j0, j1, j2, nI, nJ, SI);
         dl += dd(B.gv.JA, c.JA) + dd(B.gv.taua, c.taua) + dd(B.gv.taub, c.taub) + dd(B.gv.e, c.e) + B.gv.invAri * c.invAri.d;
-        sd = integrate_sing_vertex(vi[s1], vi[(s1 + 1) % 3], vi[(s1 + 2) % 3], vj[s2], vj[(s2 + 1) % 3], vj[(s2 + 2) % 3],
-                                   nI, nJ, SI);
+        sd = integrate_sing_vertex(i0, i1, i2v, j0, j1, j2, nI, nJ, SI);
     }
     dl += gb.x * sd.x.d + gb.y * sd.y.d + gb.z * sd.z.d + gb.w * sd.w.d;
     const int vid = slot == 0 ? I.v0 : slot == 1 ? I.v1 : slot == 2 ? I.v2 : slot == 3 ? J.v0 : slot == 4 ? J.v1 : J.v2;
