@@ -61,8 +61,8 @@ POINTS = 13
 EXECUTED_FLOPS_PER_POINT = 274
 # HBM bytes per sweep dispatch on this workload, measured with rocprofv3
 # FETCH_SIZE and WRITE_SIZE and scaled by a 1 GiB copy's readings
-# (644 MB read + 2.63 GB written; the algorithmic figure is 1.70 GB)
-SWEEP_HBM_BYTES = 3.27e9
+# (821 MB read + 3.60 GB written; the algorithmic figure is 2.33 GB)
+SWEEP_HBM_BYTES = 4.42e9
 
 
 def flops_per_point():
@@ -232,8 +232,8 @@ def main():
     points = N_DISJOINT * POINTS * 5      # level 0 (13 points) and level 1 (4 x 13 points)
     achieved = points * flops_per_point() / (sweep_ms * 1e-3) / 1e12
     # algorithmic HBM bytes of the sweep: 192-byte panel records of both panels and
-    # the per-pair outputs (two double4 and the convergence level)
-    traffic_bytes = N_DISJOINT * (2 * 32 + 1) + 2 * tri.shape[0] * 192
+    # the per-pair outputs (two double4, J and the convergence level)
+    traffic_bytes = N_DISJOINT * (2 * 32 + 24 + 1) + 2 * tri.shape[0] * 192
     Ks = {i2.TYPE_NAMES[t]: fast.refinementIterations(t) for t in range(3)}
 
     out = {
