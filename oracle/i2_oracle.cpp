@@ -8,7 +8,10 @@
 // classification by shared vertex indices); the per-type adaptive loop
 // restates src/evaluators/evaluatorJ3DK.cu:895-1012 together with
 // src/NumericalIntegrator3d.cu:37-173,400-499 and
-// src/evaluators/evaluator3d.cu:76-99,297-341.
+// src/evaluators/evaluator3d.cu:76-99,297-341; o_refinement_counts restates
+// the determineCellsToBeRefined call sequence (evaluatorJ3DK.cu:962-1010,
+// NumericalIntegrator3d.cu:461-499) and is pinned by its closed forms
+// (tests/test_cpu_api.py).
 #include "i2_oracle.hpp"
 
 #include <omp.h>

@@ -11,9 +11,11 @@
 // formula it restates; operation order follows the reference expression so
 // that rounding stays close to it.
 //
-// Parity status of each part (how it is pinned) is listed in DESIGN.md of the
-// repository; functions whose only pin is the brute-force cubature carry the
-// note "pinned by cubature" below.
+// Every part is pinned by a test under tests/ against what fixes its result
+// independently of this code (the reference's literals, closed forms,
+// invariants, brute force on small inputs, high-accuracy cubature of the
+// reference's example meshes); functions whose only pin is the brute-force
+// cubature carry the note "pinned by cubature" below.
 #pragma once
 #include <cmath>
 #include <cstdint>
