@@ -23,7 +23,7 @@ def reported_levels(conv, K, quirk, fixed=False):
     return r.astype(np.int8)
 
 
-def translation_noise(orc, i2, t, tasks, rlevel, results=False, nshift=4):
+def translation_noise(orc, i2, t, tasks, rlevel, results=False, nshift=4, rule=None):
     """Spread of the oracle's values under whole-mesh translations by ~1e-3:
     the integrals and J are translation invariant, so the spread is the FP64
     sensitivity of the reference's formulas to the last bits of the
@@ -35,7 +35,8 @@ def translation_noise(orc, i2, t, tasks, rlevel, results=False, nshift=4):
     if not hasattr(orc, "_shifted"):
         rng = np.random.default_rng(12345)
         orc._shifted = [oracle_mod.Oracle(orc.xyz + rng.uniform(-1e-3, 1e-3, size=3), orc.tri) for _ in range(nshift)]
-    args = (t, tasks, rlevel, i2.QF3D13_L12, i2.QF3D13_W, i2.QF3D13_ORDER)
+    L12, W, order = rule if rule is not None else (i2.QF3D13_L12, i2.QF3D13_W, i2.QF3D13_ORDER)
+    args = (t, tasks, rlevel, L12, W, order)
     I0, J0 = orc.values_at_levels(*args)
     nu = np.zeros(len(tasks))
     for o in orc._shifted:
@@ -44,7 +45,7 @@ def translation_noise(orc, i2, t, tasks, rlevel, results=False, nshift=4):
     return nu
 
 
-def parity_gate(orc, i2, t, tasks, gpu, cpu, rlevel, label, tol=TOL, results=False):
+def parity_gate(orc, i2, t, tasks, gpu, cpu, rlevel, label, tol=TOL, results=False, rule=None):
     """1e-11 norm-wise relative error per ordered pair.  A task above it must
     lie within 1e-11 + 4*nu of the reference formula's value in FP64 (the
     oracle) or in long double (the oracle's exact-arithmetic proxy), nu being
@@ -53,17 +54,18 @@ def parity_gate(orc, i2, t, tasks, gpu, cpu, rlevel, label, tol=TOL, results=Fal
     translations of the mesh (last bits of the coordinates) and its spread
     under Monte Carlo arithmetic (every operation and library call randomly
     perturbed by up to one ulp, which is what a different compiler or math
-    library does)."""
+    library does).  rule = (L12, weights, order), the 13-point rule if None."""
+    L12, W, order = rule if rule is not None else (i2.QF3D13_L12, i2.QF3D13_W, i2.QF3D13_ORDER)
     e = rel_err_rows(gpu, cpu)
     bad = np.nonzero(e > tol)[0]
     nu = np.zeros(len(e))
     if len(bad):
         tb, rb = np.asarray(tasks)[bad], np.asarray(rlevel)[bad]
         nu[bad] = np.maximum.reduce([
-            orc.task_noise(t, tb, rb, i2.QF3D13_L12, i2.QF3D13_W, i2.QF3D13_ORDER, results=results),
-            orc.task_noise_mca(t, tb, rb, i2.QF3D13_L12, i2.QF3D13_W, i2.QF3D13_ORDER, results=results),
-            translation_noise(orc, i2, t, tb, rb, results=results)])
-        I_ld, J_ld = orc.values_at_levels(t, tb, rb, i2.QF3D13_L12, i2.QF3D13_W, i2.QF3D13_ORDER, long_double=True)
+            orc.task_noise(t, tb, rb, L12, W, order, results=results),
+            orc.task_noise_mca(t, tb, rb, L12, W, order, results=results),
+            translation_noise(orc, i2, t, tb, rb, results=results, rule=rule)])
+        I_ld, J_ld = orc.values_at_levels(t, tb, rb, L12, W, order, long_double=True)
         e_ld = rel_err_rows(np.asarray(gpu)[bad], J_ld if results else I_ld)
         e[bad] = np.minimum(e[bad], e_ld)
     bound = tol + 4.0 * nu
