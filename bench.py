@@ -122,10 +122,24 @@ def batch64_step():
     ms = b.run()
     wall = time.perf_counter() - t0
     n = b.taskCount()
+    del b
+    # the same 64 meshes one after another through the plain per-type, per-level
+    # path (per-pair outputs), device time of each type summed
+    progress("batch64: plain path, mesh by mesh")
+    plain_ms = 0.0
+    for mesh in meshes:
+        it = i2.NumericalIntegrator3D(mesh)
+        it.setErrorControl(3, 1e-8, True)
+        ev = i2.EvaluatorJ3DK(mesh, it)
+        ev.setPath(0)
+        ev.runAllPairs(False)
+        plain_ms += sum(ev.typeStats(t)[0] for t in range(3))
+        del ev, it
     return {"workload": "64 icospheres, levels U{2,3,4,5} (seed 0): " + ",".join(str(int(x)) for x in levels) +
                         "; semi-axes U(0.5, 3.0); jitter 0.05 mean edge (seed 1000 + mesh); refinement up to 3 "
                         "levels at tol 1e-8; row-sum outputs",
             "ordered_pairs": int(n), "ms": ms, "wall_ms": 1e3 * wall, "value": n / wall, "unit": "pair-integrals/s",
+            "plain_ms": plain_ms, "plain_value": n / (plain_ms * 1e-3), "batched_over_plain": plain_ms / ms,
             "tuning": [P, ppl], "autotune_table_ms": {f"{k[0]}x{k[1]}": round(v, 2) for k, v in grid.items()}}
 
 
