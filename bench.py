@@ -286,6 +286,17 @@ def main():
                      "executed_frac": points * EXECUTED_FLOPS_PER_POINT / (sweep_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
     }
     out["autotune"] = tune
+    # the reference's (i, j)/(j, i) symmetry check (runAllPairs(true)), outside the
+    # timed steps: largest delta per type and reported level of the batched run
+    fast.runAllPairs(True)
+    delta = {}
+    for t in range(3):
+        r = fast.results(t, errors=True)
+        K = fast.refinementIterations(t)
+        c = r["conv"].astype(np.int64)
+        rl = np.where((c >= 1) & (((K - c) & 1) == 1), c - 1, c)
+        delta[i2.TYPE_NAMES[t]] = {f"level {int(L)}": float(r["errors"][rl == L].max()) for L in np.unique(rl)}
+    out["symmetry_delta_max"] = delta
     progress("adjoint")
     out["adjoint"] = adjoint_step(args)
     if not args.no_batch64:
