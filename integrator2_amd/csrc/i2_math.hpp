@@ -162,7 +162,10 @@ I2_HD __forceinline__ double log_tab(double x, const MathTable* __restrict__ tab
     if (!pos_normal(x)) return log_slow(x);
     int k;
     const double m = frexp(x, &k);                 // x = m 2^k, m in [1/2, 1)
-    const int j = (int)fma_k(m, 512.0, -255.5);    // nearest centre (ties upward), j in [0, 256]
+    // nearest centre (ties upward), j in [0, 256]: 512 m + 0.5 and 512 m - 255.5
+    // are both exact, so this equals (int)(512 m - 255.5); 0.5 is an inline
+    // constant of the instruction where -255.5 needs a register pair
+    const int j = (int)fma(m, 512.0, 0.5) - 256;
     const double r = fma(m, tab->invc[j], -1.0);   // |r| <= 2^-9
     // log1p(r) = r - r^2/2 + r^3/3 - r^4/4 + r^5/5 - r^6/6 (truncation below 2^-56 relative),
     // as r + r^2 (-1/2 + (r/3)(1 - 3r/4 + 3r^2/5 - r^3/2)): the constants 1/3 and
