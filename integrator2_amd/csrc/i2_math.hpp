@@ -202,7 +202,9 @@ I2_HD __forceinline__ double atan2_fast(double y, double x, const MathTable* __r
     double r = b.hi + (au + b.lo);                   // atan t, t in [0, 1]
     if (swap) r = PIO2_HI - r + PIO2_LO;             // atan(1/t) = pi/2 - atan t
     if (x < 0.0) r = PI_HI - r + PI_LO;              // second and third quadrants
-    return (y < 0.0 || (y == 0.0 && signbit(y))) ? -r : r;
+    // r >= 0 here, and atan2 takes the sign of y (for y = -0 as well): one
+    // bit operation instead of compares and a branch
+    return copysign(r, y);
 }
 
 }  // namespace i2m
