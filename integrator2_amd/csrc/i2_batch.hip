@@ -137,7 +137,9 @@ __device__ __forceinline__ q4 quad_part(const RuleLDS& R, int npts, int s, int s
 // Quadrature points of a disjoint item's control panel K_i (uniform over the
 // workgroup): the level-0 panel and its four level-1 children, in the order
 // sweep_task sums them, computed once per item instead of once per pair.
-struct PtsLDS { double x[5 * MAX_POINTS], y[5 * MAX_POINTS], z[5 * MAX_POINTS]; };
+// The rule weight of each point is stored beside it, so that one LDS address
+// serves the point and its weight.
+struct PtsLDS { double x[5 * MAX_POINTS], y[5 * MAX_POINTS], z[5 * MAX_POINTS], w[5 * MAX_POINTS]; };
 
 __device__ __forceinline__ void load_points(PtsLDS& P, const RuleLDS& R, int npts, const Panel& I) {
     const d3 mBC = 0.5 * (I.b + I.c), mCA = 0.5 * (I.c + I.a), mAB = 0.5 * (I.a + I.b);
@@ -151,7 +153,7 @@ __device__ __forceinline__ void load_points(PtsLDS& P, const RuleLDS& R, int npt
         d3 p = R.L1[k] * a;
         p = p + R.L2[k] * b;
         p = p + R.L3[k] * c;
-        P.x[t] = p.x; P.y[t] = p.y; P.z[t] = p.z;
+        P.x[t] = p.x; P.y[t] = p.y; P.z[t] = p.z; P.w[t] = R.w[k];
     }
 }
 
@@ -161,7 +163,7 @@ __device__ __forceinline__ q4 quad_pts(const RuleLDS& R, const PtsLDS& P, int ba
     q4 acc = q4zero();
     for (int k = s; k < npts; k += step) {
         const d3 p{P.x[base + k], P.y[base + k], P.z[base + k]};
-        acc = acc + R.w[k] * theta_psi_fast(p, J, &R.lt);
+        acc = acc + P.w[base + k] * theta_psi_fast(p, J, &R.lt);
     }
     return acc;
 }
