@@ -656,7 +656,7 @@ int i2_evaluator_loss_and_grad(i2_evaluator* ev, uint64_t seed, double* loss, do
     int K[3];
     for (int t = 0; t < 3; ++t) {
         K[t] = ev->type[t].K;
-        if (ev->path == 1) {
+        if (ev->ran_path == 1) {
             const double4* ints;
             const double* res;
             engine_evaluator_outputs(ev, t, &ints, &res, &conv[t]);

@@ -330,6 +330,7 @@ int evaluator_run(i2_evaluator* ev, bool all_pairs, bool check) {
     }
     I2_HIP(hipStreamSynchronize(ev->stream));
     ev->ran = true;
+    ev->ran_path = 0;
     ev->integ->last_ev = ev;
     return I2_OK;
 }
@@ -510,6 +511,7 @@ int i2_evaluator_run_all_pairs(i2_evaluator* ev, int checkCorrectness) {
     }
     I2_HIP(hipStreamSynchronize(ev->stream));
     ev->ran = true;
+    ev->ran_path = 1;
     ev->integ->last_ev = ev;
     return I2_OK;
 }
@@ -596,7 +598,7 @@ int i2_evaluator_get_results(const i2_evaluator* ev, int type, int32_t* tasks3, 
     const double4* d_ints = ts.buf[ts.final_buf].p;
     const double* d_res = ts.results.p;
     const int8_t* d_conv = ts.conv.p;
-    if (ev->path == 1 && ev->all_pairs) engine_evaluator_outputs(ev, type, &d_ints, &d_res, &d_conv);
+    if (ev->ran_path == 1) engine_evaluator_outputs(ev, type, &d_ints, &d_res, &d_conv);
     if (integrals4) I2_HIP(hipMemcpy(integrals4, d_ints, sizeof(double4) * n, hipMemcpyDeviceToHost));
     if (results3) I2_HIP(hipMemcpy(results3, d_res, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
     if (errors) {
@@ -630,7 +632,7 @@ int i2_evaluator_set_profiling(i2_evaluator* ev, int enable) {
 
 int i2_evaluator_sweep_ms(const i2_evaluator* ev, double* ms) {
     if (!ev || !ms) { set_error("null argument"); return I2_ERR_INVALID; }
-    if (!ev->ran || ev->path != 1 || !ev->profile) { set_error("no profiled batched run"); return I2_ERR_STATE; }
+    if (!ev->ran || ev->ran_path != 1 || !ev->profile) { set_error("no profiled batched run"); return I2_ERR_STATE; }
     *ms = ev->fast_sweep_ms;
     return I2_OK;
 }

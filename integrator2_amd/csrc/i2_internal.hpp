@@ -211,6 +211,7 @@ struct i2_evaluator {
     hipStream_t stream = nullptr;
     bool ran = false;
     bool all_pairs = false;
+    int ran_path = 0;              // path of the last run (its outputs are the ones to read)
     // the runAllPairs task lists are on the device for this mesh generation
     bool all_pairs_lists = false;
     uint64_t all_pairs_generation = 0;

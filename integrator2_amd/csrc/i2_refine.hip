@@ -84,7 +84,7 @@ static int refinement_counts(i2_evaluator* ev, int type) {
     I2_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)nT, ev->stream));
     const int8_t* conv = ts.conv.p;
     const uint8_t* unconv = ts.unconv.p;
-    if (ev->path == 1 && ev->all_pairs) {
+    if (ev->ran_path == 1) {
         const double4* ints;
         const double* res;
         engine_evaluator_outputs(ev, type, &ints, &res, &conv);

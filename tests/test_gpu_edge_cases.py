@@ -52,3 +52,30 @@ def test_plain_path_refuses_more_than_32_bit_tasks(gpu):
     ev.setPath(0)
     with pytest.raises(gpu.I2Error, match="32-bit"):
         ev.runAllPairs(False)
+
+
+def test_results_follow_the_path_of_the_last_run(gpu):
+    """Changing the path after a run does not change what the run's getters read."""
+    xyz, tri = gpu.icosphere(1, (2.0, 1.0, 0.7), 0.05, 2)
+    m = gpu.Mesh3D()
+    m.setData(xyz, tri)
+    m.prepareMesh()
+    it = gpu.NumericalIntegrator3D(m)
+    it.setErrorControl(2, 1e-8)
+    ev = gpu.EvaluatorJ3DK(m, it)
+    ev.setPath(1)
+    ev.runAllPairs(True)
+    batched = [ev.results(t, errors=True) for t in range(3)]
+    counts = [it.getRefinementsRequired(t) for t in range(3)]
+    ev.setPath(0)
+    for t in range(3):
+        r = ev.results(t, errors=True)
+        for key in ("tasks", "integrals", "results", "conv", "errors"):
+            assert np.array_equal(r[key], batched[t][key]), (t, key)
+        assert np.array_equal(it.getRefinementsRequired(t), counts[t])
+    ev.runAllPairs(True)
+    for t in range(3):
+        r = ev.results(t)
+        assert np.array_equal(r["conv"], batched[t]["conv"])
+        e = np.abs(r["integrals"] - batched[t]["integrals"]).max(axis=1) / np.abs(batched[t]["integrals"]).max(axis=1)
+        assert e.max() <= 1e-11, (t, e.max())
