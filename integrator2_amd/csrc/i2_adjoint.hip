@@ -21,9 +21,12 @@
 //   k_adj_deep      disjoint pairs at reported levels >= 2 (queued by the
 //                   sweep): one wavefront per pair over its 13 * 4^L points.
 //   k_adj_singular  vertex- and edge-sharing pairs: one wavefront per pair;
-//                   reverse mode over the points, then forward-mode dual
-//                   numbers along the 18 vertex coordinates of the pair for
-//                   the per-pair geometry, singular-part contexts and the
+//                   reverse mode over the points; the cotangents of the
+//                   per-pair quantities go to a record per pair.
+//   k_adj_singular_dual  the per-pair chain of every record: forward-mode
+//                   dual numbers along the 18 vertex coordinates of the pair
+//                   (one per lane, 3.5 pairs per wavefront) through the
+//                   per-pair geometry, singular-part contexts and the
 //                   closed-form singular integrals.
 #include "i2_adjoint.hpp"
 #include "i2_adjoint_point.hpp"
@@ -363,8 +366,8 @@ struct SingBars {
 };
 
 template <int TYPE>
-__device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R, const AdjMesh& M, int i, int j, int level,
-                                              int lane) {
+__device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R, const AdjMesh& M, int64_t t, int i, int j,
+                                              int level, int lane) {
     const Panel I = uniform_panel(M.panels[i]);   // one wavefront per pair
     const Panel J = uniform_panel(M.panels[j]);
     const Weights W = weights(A.seed, i, j, M.nT);
@@ -435,9 +438,35 @@ __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R,
     const double thetaw = Iv.w + 2.0 * pw * (TWO_PI * I.area);
     const d3 nb = RECIP_4PI * (thetaw * W.v + cross(W.v, psi));
     const double Sb = dot4(gb, B.F) * (fa / I.area) + (TYPE == I2_SIMPLE_NEIGHBORS ? pw * dot(W.v, J.n) : 0.0);
-    // forward mode along coordinate d = 3 * slot + axis, slots 0..2 = K_i's, 3..5 = K_j's vertices
-    if (lane >= 18) return;
-    const int slot = lane / 3, axis = lane % 3;
+    // the per-task chain runs in k_adj_singular_dual, one coordinate per lane
+    if (lane == 0) {
+        SingRec& r = A.sing_rec[t];
+        r.i = i; r.j = j; r.s1 = s1; r.s2 = s2;
+        r.g[0] = B.g.a; r.g[1] = B.g.b; r.g[2] = B.g.c; r.g[3] = B.g.ta; r.g[4] = B.g.tb; r.g[5] = B.g.tc;
+        if (TYPE == I2_ATTACHED_NEIGHBORS) {
+            r.cb[0] = B.ge.JB; r.cb[1] = B.ge.JC; r.cb[2] = B.ge.taua; r.cb[3] = B.ge.taub; r.cb[4] = B.ge.tauc;
+            r.cs = B.ge.ilvc;
+        } else {
+            r.cb[0] = B.gv.JA; r.cb[1] = B.gv.taua; r.cb[2] = B.gv.taub; r.cb[3] = B.gv.e; r.cb[4] = {0.0, 0.0, 0.0};
+            r.cs = B.gv.invAri;
+        }
+        r.X[0] = B.X[0]; r.X[1] = B.X[1]; r.X[2] = B.X[2];
+        r.nb = nb;
+        r.Sb = Sb;
+        r.gb = gb;
+    }
+}
+
+// Forward mode along coordinate d = 3 * slot + axis (slots 0..2 = K_i's,
+// 3..5 = K_j's vertices) of a singular task's per-task chain: panel normals,
+// area and tangents, the singular context and the closed-form singular
+// integral, weighted by the cotangents the point loop left in the record.
+template <int TYPE>
+__device__ __forceinline__ void singular_chain(const AdjMesh& M, const SingRec& r, int d) {
+    const Panel I = M.panels[r.i];
+    const Panel J = M.panels[r.j];
+    const int s1 = r.s1, s2 = r.s2;
+    const int slot = d / 3, axis = d % 3;
     const D3 ia = dual_vertex(I.a, 0, slot, axis), ib = dual_vertex(I.b, 1, slot, axis), ic = dual_vertex(I.c, 2, slot, axis);
     const D3 ja = dual_vertex(J.a, 3, slot, axis), jb = dual_vertex(J.b, 4, slot, axis), jc = dual_vertex(J.c, 5, slot, axis);
     const D3 cri = cross(ib - ia, ic - ia), crj = cross(jb - ja, jc - ja);
@@ -447,25 +476,21 @@ __device__ __forceinline__ void singular_task(const AdjArgs& A, const RuleSh& R,
     const D3 i0 = pick3(s1, ia, ib, ic), i1 = pick3((s1 + 1) % 3, ia, ib, ic), i2v = pick3((s1 + 2) % 3, ia, ib, ic);
     const D3 j0 = pick3(s2, ja, jb, jc), j1 = pick3((s2 + 1) % 3, ja, jb, jc), j2 = pick3((s2 + 2) % 3, ja, jb, jc);
     double dl = 0.0;
-    if (slot < 3) {
-        const d3 gx = pick3(slot, B.X[0], B.X[1], B.X[2]);
-        dl += (axis == 0 ? gx.x : (axis == 1 ? gx.y : gx.z));
-    } else {
-        const d3 gd = slot == 3 ? B.g.a : (slot == 4 ? B.g.b : B.g.c);
-        dl += (axis == 0 ? gd.x : (axis == 1 ? gd.y : gd.z));
-    }
-    dl += dd(B.g.ta, ta) + dd(B.g.tb, tb) + dd(B.g.tc, tc) + dd(nb, nJ) + Sb * SI.d;
+    const d3 gx = pick3(slot % 3, slot < 3 ? r.X[0] : r.g[0], slot < 3 ? r.X[1] : r.g[1], slot < 3 ? r.X[2] : r.g[2]);
+    dl += (axis == 0 ? gx.x : (axis == 1 ? gx.y : gx.z));
+    dl += dd(r.g[3], ta) + dd(r.g[4], tb) + dd(r.g[5], tc) + dd(r.nb, nJ) + r.Sb * SI.d;
     Q4 sd;
     if (TYPE == I2_ATTACHED_NEIGHBORS) {
         const EdgeCtxT<Dual> c = edge_ctx(j0, j1, j2);
-        dl += dd(B.ge.JB, c.JB) + dd(B.ge.JC, c.JC) + dd(B.ge.taua, c.taua) + dd(B.ge.taub, c.taub) +
-              dd(B.ge.tauc, c.tauc) + B.ge.ilvc * c.ilvc.d;
+        dl += dd(r.cb[0], c.JB) + dd(r.cb[1], c.JC) + dd(r.cb[2], c.taua) + dd(r.cb[3], c.taub) + dd(r.cb[4], c.tauc) +
+              r.cs * c.ilvc.d;
         sd = integrate_sing_edge(i0, i1, i2v, j0, j1, j2, nI, nJ, SI);
     } else {
         const VertexCtxT<Dual> c = vertex_ctx(j0, j1, j2, nI, nJ, SI);
-        dl += dd(B.gv.JA, c.JA) + dd(B.gv.taua, c.taua) + dd(B.gv.taub, c.taub) + dd(B.gv.e, c.e) + B.gv.invAri * c.invAri.d;
+        dl += dd(r.cb[0], c.JA) + dd(r.cb[1], c.taua) + dd(r.cb[2], c.taub) + dd(r.cb[3], c.e) + r.cs * c.invAri.d;
         sd = integrate_sing_vertex(i0, i1, i2v, j0, j1, j2, nI, nJ, SI);
     }
+    const q4 gb = r.gb;
     dl += gb.x * sd.x.d + gb.y * sd.y.d + gb.z * sd.z.d + gb.w * sd.w.d;
     const int vid = slot == 0 ? I.v0 : slot == 1 ? I.v1 : slot == 2 ? I.v2 : slot == 3 ? J.v0 : slot == 4 ? J.v1 : J.v2;
     atomicAdd(M.dX + 3 * (int64_t)vid + axis, dl);
@@ -488,8 +513,23 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_adj_singular(AdjArgs A) {
         int i = L[3 * k], j = L[3 * k + 1];
         if (z >= n) { const int s = i; i = j; j = s; }
         const int level = reported_level(A, M.conv[type][z], M.K[type]);
-        if (type == I2_ATTACHED_NEIGHBORS) singular_task<I2_ATTACHED_NEIGHBORS>(A, R, M, i, j, level, lane);
-        else singular_task<I2_SIMPLE_NEIGHBORS>(A, R, M, i, j, level, lane);
+        if (type == I2_ATTACHED_NEIGHBORS) singular_task<I2_ATTACHED_NEIGHBORS>(A, R, M, t, i, j, level, lane);
+        else singular_task<I2_SIMPLE_NEIGHBORS>(A, R, M, t, i, j, level, lane);
+    }
+}
+
+// The per-task chains of all singular tasks: 18 coordinates per task, one per
+// lane (tasks are ordered by type, so wavefronts rarely mix the two types).
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adj_singular_dual(AdjArgs A) {
+    const int64_t total = A.singular_tasks * 18;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = g / 18;
+        const int d = (int)(g - 18 * t);
+        const int b = find_base_a(A.sing_base, 2 * A.nmeshes, t);
+        const AdjMesh& M = A.meshes[b % A.nmeshes];
+        const SingRec& r = A.sing_rec[t];
+        if (b / A.nmeshes == 0) singular_chain<I2_SIMPLE_NEIGHBORS>(M, r, d);
+        else singular_chain<I2_ATTACHED_NEIGHBORS>(M, r, d);
     }
 }
 
@@ -510,6 +550,8 @@ int adjoint_launch(AdjState& S, hipStream_t st) {
     if (S.args.singular_tasks > 0) {
         hipLaunchKernelGGL(k_adj_singular, dim3(S.grid), dim3(ADJ_BLOCK), 0, st, S.args);
         I2_TRY(check_launch("k_adj_singular"));
+        hipLaunchKernelGGL(k_adj_singular_dual, dim3(S.grid), dim3(ADJ_BLOCK), 0, st, S.args);
+        I2_TRY(check_launch("k_adj_singular_dual"));
     }
     return I2_OK;
 }
@@ -558,6 +600,8 @@ int adjoint_setup(AdjState& S, int device, std::vector<AdjMesh> D, const std::ve
     A.item_base = S.d_item_base.p;
     A.singular_tasks = sing_base[2 * M];
     A.sing_base = S.d_sing_base.p;
+    I2_TRY(S.d_sing_rec.ensure(A.singular_tasks));
+    A.sing_rec = S.d_sing_rec.p;
     I2_TRY(S.d_counters.ensure(4));
     A.counters = S.d_counters.p;
     if (S.deep_cap == 0) S.deep_cap = 1 << 16;

@@ -27,6 +27,21 @@ struct AdjMesh {
 // A disjoint pair whose reported level is >= 2.
 struct AdjEntry { int m, i, j, level; };
 
+// A vertex- or edge-sharing task after its point loop: the cotangents of the
+// parameters of its per-point functions and of the panel quantities, for the
+// per-task chain to the vertex coordinates (forward-mode dual numbers, one
+// coordinate per lane, in a kernel of its own).
+struct SingRec {
+    int i, j, s1, s2;
+    i2d::d3 g[6];       // thetaPsi: K_j vertices a, b, c and tangents ta, tb, tc
+    i2d::d3 cb[5];      // singular context: JB, JC, tau_a, tau_b, tau_c (edge) or JA, tau_a, tau_b, e (vertex)
+    double cs;          // 1/|JC - JB| (edge) or 1/sqrt(S_i) (vertex)
+    i2d::d3 X[3];       // K_i vertices through the quadrature points
+    i2d::d3 nb;         // unit normal of K_j
+    double Sb;          // area of K_i
+    i2d::q4 gb;         // the task's (Psi, Theta) cotangent
+};
+
 struct AdjArgs {
     const AdjMesh* meshes;
     int nmeshes;
@@ -44,6 +59,7 @@ struct AdjArgs {
     unsigned long long* counters; // [0] disjoint item, [1] deep pairs queued
     AdjEntry* deep;
     int64_t deep_cap;
+    SingRec* sing_rec;            // per singular task (singular_tasks entries)
 };
 
 struct AdjState {
@@ -59,6 +75,7 @@ struct AdjState {
     DBuf<int64_t> d_item_base, d_sing_base;
     DBuf<unsigned long long> d_counters;
     DBuf<AdjEntry> d_deep;
+    DBuf<SingRec> d_sing_rec;
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
