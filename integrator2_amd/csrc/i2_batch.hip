@@ -211,7 +211,8 @@ __device__ __forceinline__ void finish_task(const BatchArgs& A, const MeshDesc& 
                                             const Panel& I, const Panel& J, q4* rp0, q4* rp1) {
     q4 cur = o.cur, prev = o.prev;
     if (TYPE != I2_NOT_NEIGHBORS) {
-        const q4 s = singular_integral(TYPE, I, J);
+        const double4 s4 = M.sing_val[TYPE][z];
+        const q4 s = {s4.x, s4.y, s4.z, s4.w};
         cur = cur + s;
         prev = prev + s;
     }
@@ -536,6 +537,27 @@ __global__ __launch_bounds__(256) void k_finalize_disjoint(BatchArgs A) {
     }
 }
 
+// The closed-form singular integral of every vertex- and edge-sharing task, one
+// task per lane (the task's last kernel adds it to the level value; a refinement
+// kernel would otherwise evaluate it on one lane of each lane group).
+__global__ __launch_bounds__(256) void k_singular_closed(BatchArgs A) {
+    const int64_t total = A.singular_items * 256;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t item = g >> 8;
+        const int b = find_base(A.sing_base, 2 * A.nmeshes, item);
+        const int type = b / A.nmeshes == 0 ? I2_SIMPLE_NEIGHBORS : I2_ATTACHED_NEIGHBORS;
+        const MeshDesc& M = A.meshes[b % A.nmeshes];
+        const int64_t n = M.npairs[type];
+        const int64_t z = (item - A.sing_base[b]) * 256 + (g & 255);
+        if (z >= 2 * n) continue;
+        const int64_t k = z < n ? z : z - n;
+        int i = M.lists[type][3 * k], j = M.lists[type][3 * k + 1];
+        if (z >= n) { const int t = i; i = j; j = t; }
+        const q4 s = singular_integral(type, M.panels[i], M.panels[j]);
+        M.sing_val[type][z] = make_double4(s.x, s.y, s.z, s.w);
+    }
+}
+
 // Vertex- and edge-sharing tasks, enumerated as the singular sweep's items.
 __global__ __launch_bounds__(256) void k_finalize_singular(BatchArgs A) {
     const int64_t total = A.singular_items * 256;
@@ -652,6 +674,10 @@ int batch_enqueue(Engine& E, hipStream_t st) {
     I2_TRY(check_launch("k_zero_u64"));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, st, E.d_levelmask.p, (int64_t)E.d_levelmask.n);
     I2_TRY(check_launch("k_zero_u64"));
+    if (A.singular_items > 0) {
+        hipLaunchKernelGGL(k_singular_closed, dim3(E.deep_grid), dim3(256), 0, st, A);
+        I2_TRY(check_launch("k_singular_closed"));
+    }
     if (E.prof0) I2_HIP(hipEventRecord(E.prof0, st));
     I2_TRY(launch_sweep(E, st));
     for (int L = 2; L <= A.last_level; ++L) {
@@ -812,6 +838,7 @@ struct i2_batch {
         i2::DBuf<int8_t> conv[3];
         i2::DBuf<double> res[3];
         i2::DBuf<unsigned long long> unconv;   // per type and control panel: unconverged at maxLevel (bytes)
+        i2::DBuf<double4> sing_val[2];         // closed-form singular integrals per singular task
         i2::DBuf<double> rowacc, rowout;
     };
     std::vector<Out> outs;
@@ -862,6 +889,10 @@ static int batch_prepare(i2_batch* b) {
         if (mesh->device != b->device) { set_error("mesh %d lives on device %d, batch on %d", m, mesh->device, b->device); return I2_ERR_INVALID; }
         MeshDesc D = desc_of(mesh);
         auto& o = b->outs[m];
+        for (int t = 0; t < 2; ++t) {
+            I2_TRY(o.sing_val[t].ensure(2 * mesh->pairs[t]));
+            D.sing_val[t] = o.sing_val[t].p;
+        }
         if (b->out == OUT_PER_PAIR) {
             for (int t = 0; t < 3; ++t) {
                 const int64_t n = 2 * mesh->pairs[t];

@@ -26,6 +26,9 @@ struct MeshDesc {
     // per control panel: 1 where a task was still unconverged after the
     // comparison at maxLevel (automatic mode; nT bytes per type, may be null)
     uint8_t* unconv[3];
+    // closed-form singular integral of every vertex- and edge-sharing task
+    // (2 npairs each, reference task order), computed before the sweeps
+    double4* sing_val[2];
     // row-sum outputs: rowacc[type][parity][nT][4], rowout[nT][4]
     double* rowacc;
     double* rowout;
